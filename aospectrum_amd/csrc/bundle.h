@@ -1,0 +1,115 @@
+// Device-resident state of one operator bundle and the host pipeline API.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/aosx.h"
+#include "symbolic.h"
+
+namespace aosx {
+
+struct Error : std::runtime_error {
+    int code;
+    int hip_status;
+    Error(int c, const std::string &m, int h = 0) : std::runtime_error(m), code(c), hip_status(h) {}
+};
+
+#define AOSX_HIP(call)                                                                                 \
+    do {                                                                                               \
+        hipError_t _e = (call);                                                                        \
+        if (_e != hipSuccess)                                                                          \
+            throw ::aosx::Error(AOSX_HIP_ERROR, std::string(#call) + " failed: " + hipGetErrorString(_e), \
+                                (int)_e);                                                              \
+    } while (0)
+
+#define AOSX_LAUNCHED() AOSX_HIP(hipGetLastError())
+
+struct KernelStats {
+    bool enabled = false;
+    int64_t schur_launches = 0;
+    double schur_flops = 0.0;
+    double schur_ms = 0.0;
+    int64_t panel_launches = 0;
+    double panel_ms = 0.0;
+    int64_t getrf_launches = 0;
+    double getrf_ms = 0.0;
+    int64_t solve_launches = 0;
+    double solve_ms = 0.0;
+    double solve_bytes = 0.0;
+    int64_t assemble_launches = 0;
+    double assemble_ms = 0.0;
+    double assemble_bytes = 0.0;
+};
+
+struct CissParams {
+    int64_t nk = 0;
+    double lower = 0.0, upper = 0.0;
+    int integration_points = 32, contour_batch_size = 1, block_size = 32, moment_size = 4;
+    double vscale = 0.1, guard = 1e-5, rank_tol = 1e-5, spurious = 1e-4;
+    int precision = AOSX_PRECISION_FP32, refinement_steps = 0, refinement_passes = 0, k_batch = 1;
+    int conjugate_pairs = 0;
+};
+
+struct CissOutput {
+    std::vector<std::vector<float>> energies;  // per k, ascending
+    std::vector<double> max_residual;
+    std::vector<int> rank;
+    std::vector<int> saturated;
+    double stages[AOSX_STAGE_COUNT] = {0};
+    int64_t replaced_pivots = 0;
+    int64_t factorizations = 0;
+};
+
+struct Bundle {
+    int device = -1;
+    BlockTable blocks_view;  // valid during create only
+    Pattern pat;
+    TileStructure ts;
+    int64_t n = 0, nnz = 0, n_groups = 0;
+    hipStream_t stream = nullptr;
+    // resident device data
+    float2 *d_hval = nullptr, *d_sval = nullptr;
+    int32_t *d_source = nullptr, *d_group = nullptr, *d_starts = nullptr, *d_mirror = nullptr;
+    int32_t *d_rows = nullptr, *d_indices = nullptr, *d_indptr = nullptr, *d_shifts = nullptr;
+    int32_t *d_iperm = nullptr, *d_perm = nullptr, *d_tile_id = nullptr, *d_next = nullptr, *d_prev = nullptr;
+    uint64_t resident_bytes = 0;
+    uint64_t peak_bytes = 0;
+    KernelStats kstats;
+    struct Slot {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Slot> ws;
+
+    ~Bundle();
+    void *raw(const std::string &name, size_t bytes);
+    template <typename T>
+    T *buf(const std::string &name, size_t count) {
+        return static_cast<T *>(raw(name, count * sizeof(T)));
+    }
+    void release_workspace();
+    uint64_t workspace_bytes() const;
+    size_t factor_floats() const { return (size_t)ts.n_tiles * 2 * 64 * 64; }
+
+    void zero(void *p, size_t bytes);  // bytes: multiple of 16
+    void upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values);
+    // pipeline stages
+    void assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *d_s);
+    void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
+                float *tiles, float *dinv, int *d_replaced);
+    void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans);
+    void solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
+                    const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
+                    const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
+                    float2 *x_out, double *rel_residual);
+    void band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
+              const float2 *probes, CissOutput &out);
+};
+
+}  // namespace aosx

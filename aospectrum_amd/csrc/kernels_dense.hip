@@ -1,0 +1,423 @@
+// Tall-skinny and small dense kernels of the CISS subspace stage.
+//
+// Tall-skinny operands (n x w, row-major, complex64) come from the shifted
+// solves; the projected problems are small (w <= 256) and live in FP64:
+//   spmm        Y = A X with A a CSR H(k) or S(k) (complex64, FP32 accumulate)
+//   moments     M[:, mL:(m+1)L] += coef[b][m] X_b           (ciss.py:324-333)
+//   gram64      G = A^H B in FP64 (two-stage, deterministic order)
+//   tall_gemm   Y = X T with T small (FP64), output complex64 or complex128
+//   jacobi      Hermitian eigendecomposition, FP64, cyclic two-sided Jacobi with
+//               round-robin ordering (r/2 disjoint rotations per round)
+//   cholesky / trinv / small_gemm for the projected generalized problem
+//   eig_residual ||H v - e S v||_2 per column in FP64
+#include "device_common.h"
+
+namespace aosx {
+
+// Y[j] = A_k X[j] for the matrices j of one launch: A values a[kidx[j]],
+// X at x + xidx[j]*n*w, Y at y + j*n*w.  One wave per row, lanes over columns.
+__global__ void __launch_bounds__(256) k_spmm(float2 *__restrict__ y, const float2 *__restrict__ a,
+                                             const int32_t *__restrict__ kidx, const float2 *__restrict__ x,
+                                             const int32_t *__restrict__ xidx, int w,
+                                             const int32_t *__restrict__ indptr,
+                                             const int32_t *__restrict__ indices, int64_t n, int64_t nnz) {
+    const int j = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float2 *ak = a + (int64_t)kidx[j] * nnz;
+    const float2 *xj = x + (int64_t)xidx[j] * n * w;
+    float2 *yj = y + (int64_t)j * n * w;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < n; row += (int64_t)gridDim.x * 4) {
+        for (int c = lane; c < w; c += 64) {
+            float ar = 0.f, ai = 0.f;
+            for (int32_t p = indptr[row]; p < indptr[row + 1]; ++p) {
+                const float2 v = ak[p], xv = xj[(int64_t)indices[p] * w + c];
+                ar += v.x * xv.x - v.y * xv.y;
+                ai += v.x * xv.y + v.y * xv.x;
+            }
+            yj[row * w + c] = make_float2(ar, ai);
+        }
+    }
+}
+
+// M[row][m*L + c] += sum_b coef[b*msize + m] * X_b[row][c]   (b over nb solutions)
+__global__ void k_moments(float2 *__restrict__ M, const float2 *__restrict__ X, int nb, int L, int msize,
+                          const float2 *__restrict__ coef, int64_t n) {
+    const int64_t total = n * L, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t row = e / L;
+        const int c = (int)(e % L);
+        for (int m = 0; m < msize; ++m) {
+            float2 acc = M[row * (int64_t)(L * msize) + m * L + c];
+            for (int b = 0; b < nb; ++b) {
+                const float2 xv = X[(int64_t)b * n * L + e];
+                const float2 cf = coef[b * msize + m];
+                acc.x += cf.x * xv.x - cf.y * xv.y;
+                acc.y += cf.x * xv.y + cf.y * xv.x;
+            }
+            M[row * (int64_t)(L * msize) + m * L + c] = acc;
+        }
+    }
+}
+
+// Partial Gram: part[k][chunk][i][j] = sum_{rows of chunk} conj(A[row][i]) B[row][j]
+// over a 32x32 output tile (grid.x tiles, grid.y chunks, grid.z problems).
+__global__ void __launch_bounds__(256) k_gram_partial(c64 *__restrict__ part, const float2 *__restrict__ A,
+                                                     int64_t a_stride, int P, const float2 *__restrict__ B,
+                                                     int64_t b_stride, int Q, int64_t n, int rows_per_chunk) {
+    __shared__ float2 As[32][33], Bs[32][33];
+    const int k = blockIdx.z, chunk = blockIdx.y, tid = threadIdx.x;
+    const int tiles_q = (Q + 31) / 32;
+    const int i0 = (blockIdx.x / tiles_q) * 32, j0 = (blockIdx.x % tiles_q) * 32;
+    const float2 *a = A + k * a_stride;
+    const float2 *b = B + k * b_stride;
+    const int64_t r_begin = (int64_t)chunk * rows_per_chunk;
+    const int64_t r_end = min(n, r_begin + rows_per_chunk);
+    const int ti = tid >> 3, tj = (tid & 7) * 4;
+    double accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += 32) {
+        for (int e = tid; e < 32 * 32; e += 256) {
+            const int rr = e >> 5, cc = e & 31;
+            const int64_t row = r0 + rr;
+            float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+            if (row < r_end) {
+                if (i0 + cc < P) va = a[row * P + i0 + cc];
+                if (j0 + cc < Q) vb = b[row * Q + j0 + cc];
+            }
+            As[rr][cc] = va;
+            Bs[rr][cc] = vb;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < 32; ++rr) {
+            const float2 av = As[rr][ti];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float2 bv = Bs[rr][tj + q];
+                accr[q] += (double)av.x * bv.x + (double)av.y * bv.y;
+                acci[q] += (double)av.x * bv.y - (double)av.y * bv.x;
+            }
+        }
+        __syncthreads();
+    }
+    c64 *out = part + ((int64_t)k * gridDim.y + chunk) * (int64_t)P * Q;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + ti, j = j0 + tj + q;
+        if (i < P && j < Q) out[(int64_t)i * Q + j] = {accr[q], acci[q]};
+    }
+}
+
+// G[k][i][j] (leading dimension ldg) = sum over chunks of part; hermitian=1
+// also symmetrises (G + G^H)/2 (requires P == Q).
+__global__ void k_gram_reduce(c64 *__restrict__ G, int ldg, int64_t g_stride, const c64 *__restrict__ part,
+                              int P, int Q, int chunks, int hermitian) {
+    const int k = blockIdx.y;
+    const c64 *pk = part + (int64_t)k * chunks * P * Q;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < P * Q; e += gridDim.x * blockDim.x) {
+        const int i = e / Q, j = e % Q;
+        c64 s = {0.0, 0.0};
+        for (int c = 0; c < chunks; ++c) s = cadd(s, pk[(int64_t)c * P * Q + e]);
+        if (hermitian) {
+            c64 t = {0.0, 0.0};
+            for (int c = 0; c < chunks; ++c) t = cadd(t, pk[(int64_t)c * P * Q + (int64_t)j * Q + i]);
+            s = {(s.re + t.re) * 0.5, (s.im - t.im) * 0.5};
+        }
+        G[k * g_stride + (int64_t)i * ldg + j] = s;
+    }
+}
+
+// Y[k][row][j] = sum_i X[k][row][i] T[k][i][j]  (T leading dimension ldt)
+template <typename Out>
+__global__ void __launch_bounds__(256) k_tall_gemm(Out *__restrict__ Y, int64_t y_stride,
+                                                  const float2 *__restrict__ X, int64_t x_stride, int P,
+                                                  const c64 *__restrict__ T, int ldt, int64_t t_stride, int Q,
+                                                  int64_t n) {
+    const int k = blockIdx.z;
+    const int j = blockIdx.y * 32 + (threadIdx.x & 31);
+    const int64_t row = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    if (row >= n || j >= Q) return;
+    const float2 *x = X + k * x_stride + row * P;
+    const c64 *t = T + k * t_stride;
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < P; ++i) {
+        const float2 xv = x[i];
+        const c64 tv = t[(int64_t)i * ldt + j];
+        ar += xv.x * tv.re - xv.y * tv.im;
+        ai += xv.x * tv.im + xv.y * tv.re;
+    }
+    Out o;
+    o.x = ar;
+    o.y = ai;
+    Y[k * y_stride + row * Q + j] = o;
+}
+template __global__ void k_tall_gemm<float2>(float2 *, int64_t, const float2 *, int64_t, int, const c64 *, int,
+                                             int64_t, int, int64_t);
+template __global__ void k_tall_gemm<double2>(double2 *, int64_t, const float2 *, int64_t, int, const c64 *,
+                                              int, int64_t, int, int64_t);
+
+// ---------------------------------------------------------------- Jacobi
+// One workgroup per problem.  A (r x r, leading dimension ld) is destroyed;
+// on exit w holds ascending eigenvalues and Vout the matching eigenvectors
+// (columns).  Rotation for the pivot pair (p, q) with b = A_pq = |b| e^{i phi}:
+// G = diag(1, e^{-i phi}) [[c, s], [-s, c]] with the real Jacobi angle of
+// [[a_pp, |b|], [|b|, a_qq]] (Golub & Van Loan, sym.schur2); A <- G^H A G.
+__global__ void __launch_bounds__(1024) k_jacobi(c64 *__restrict__ A_all, c64 *__restrict__ V_all,
+                                                c64 *__restrict__ Vout_all, double *__restrict__ w_all,
+                                                const int32_t *__restrict__ rk, int ld, int64_t stride,
+                                                int max_sweeps, double tol, int *__restrict__ sweeps_out) {
+    __shared__ double rc[128], rs[128], rpr[128], rpi[128];
+    __shared__ int rp[128], rq[128];
+    __shared__ double red[1024], red2[1024];
+    __shared__ int done;
+    const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    const int r = rk[k];
+    c64 *A = A_all + k * stride;
+    c64 *V = V_all + k * stride;
+    c64 *Vout = Vout_all + k * stride;
+    double *w = w_all + (int64_t)k * ld;
+    for (int e = tid; e < r * r; e += nthr) {
+        const int i = e / r, j = e % r;
+        V[(int64_t)i * ld + j] = {i == j ? 1.0 : 0.0, 0.0};
+    }
+    const int rr = r + (r & 1);
+    const int npairs = rr / 2;
+    int sweep = 0;
+    __syncthreads();
+    for (; sweep < max_sweeps; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < r * r; e += nthr) {
+            const int i = e / r, j = e % r;
+            const double v = cabs2(A[(int64_t)i * ld + j]);
+            tot += v;
+            if (i != j) off += v;
+        }
+        red[tid] = off;
+        red2[tid] = tot;
+        __syncthreads();
+        for (int s2 = nthr / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) { red[tid] += red[tid + s2]; red2[tid] += red2[tid + s2]; }
+            __syncthreads();
+        }
+        if (tid == 0) done = !(red[0] > tol * tol * red2[0]);
+        __syncthreads();
+        if (done) break;
+        for (int t = 0; t < rr - 1; ++t) {
+            if (tid < npairs) {
+                int p, q;
+                if (tid == 0) { p = rr - 1; q = t; }
+                else { p = (t + tid) % (rr - 1); q = (t - tid + (rr - 1)) % (rr - 1); }
+                if (p > q) { const int tmp = p; p = q; q = tmp; }
+                double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
+                if (q < r) {
+                    const c64 b = A[(int64_t)p * ld + q];
+                    const double mb = sqrt(cabs2(b));
+                    const double app = A[(int64_t)p * ld + p].re, aqq = A[(int64_t)q * ld + q].re;
+                    if (mb > 1e-300 && mb > 1e-18 * sqrt(fabs(app * aqq))) {
+                        const double tau = (aqq - app) / (2.0 * mb);
+                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + tt * tt);
+                        s = tt * c;
+                        er = b.re / mb;
+                        ei = b.im / mb;  // e^{i phi}
+                    }
+                } else {
+                    q = p;  // dummy partner: no rotation
+                }
+                rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = s; rpr[tid] = er; rpi[tid] = ei;
+            }
+            __syncthreads();
+            // columns: A <- A G, V <- V G
+            for (int e = tid; e < r * npairs; e += nthr) {
+                const int i = e / npairs, pi = e % npairs;
+                const int p = rp[pi], q = rq[pi];
+                if (p == q || rs[pi] == 0.0) continue;
+                const double c = rc[pi], s = rs[pi];
+                const c64 em = {rpr[pi], -rpi[pi]};  // e^{-i phi}
+                c64 *ap = A + (int64_t)i * ld + p, *aq = A + (int64_t)i * ld + q;
+                const c64 x = *ap, y = cmul(em, *aq);
+                *ap = {c * x.re - s * y.re, c * x.im - s * y.im};
+                *aq = {s * x.re + c * y.re, s * x.im + c * y.im};
+                c64 *vp = V + (int64_t)i * ld + p, *vq = V + (int64_t)i * ld + q;
+                const c64 xv = *vp, yv = cmul(em, *vq);
+                *vp = {c * xv.re - s * yv.re, c * xv.im - s * yv.im};
+                *vq = {s * xv.re + c * yv.re, s * xv.im + c * yv.im};
+            }
+            __syncthreads();
+            // rows: A <- G^H A
+            for (int e = tid; e < r * npairs; e += nthr) {
+                const int pi = e / r, j = e % r;
+                const int p = rp[pi], q = rq[pi];
+                if (p == q || rs[pi] == 0.0) continue;
+                const double c = rc[pi], s = rs[pi];
+                const c64 ep = {rpr[pi], rpi[pi]};  // e^{+i phi}
+                c64 *ap = A + (int64_t)p * ld + j, *aq = A + (int64_t)q * ld + j;
+                const c64 x = *ap, y = cmul(ep, *aq);
+                *ap = {c * x.re - s * y.re, c * x.im - s * y.im};
+                *aq = {s * x.re + c * y.re, s * x.im + c * y.im};
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0 && sweeps_out) sweeps_out[k] = sweep;
+    // ascending order by rank counting (stable)
+    for (int i = tid; i < r; i += nthr) {
+        const double wi = A[(int64_t)i * ld + i].re;
+        int rank = 0;
+        for (int j = 0; j < r; ++j) {
+            const double wj = A[(int64_t)j * ld + j].re;
+            rank += (wj < wi) || (wj == wi && j < i);
+        }
+        w[rank] = wi;
+        for (int row = 0; row < r; ++row) Vout[(int64_t)row * ld + rank] = V[(int64_t)row * ld + i];
+    }
+}
+
+// Lower Cholesky in place (upper part zeroed); status[k] = 1 when not positive definite.
+__global__ void __launch_bounds__(1024) k_cholesky(c64 *__restrict__ A_all, const int32_t *__restrict__ rk,
+                                                  int ld, int64_t stride, int *__restrict__ status) {
+    __shared__ int bad;
+    const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    const int r = rk[k];
+    c64 *A = A_all + k * stride;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < r; ++j) {
+        if (tid == 0) {
+            const double d = A[(int64_t)j * ld + j].re;
+            if (!(d > 0.0) || !isfinite(d)) bad = 1;
+            else A[(int64_t)j * ld + j] = {sqrt(d), 0.0};
+        }
+        __syncthreads();
+        if (bad) break;
+        const double djj = A[(int64_t)j * ld + j].re;
+        for (int i = j + 1 + tid; i < r; i += nthr) {
+            c64 &v = A[(int64_t)i * ld + j];
+            v = {v.re / djj, v.im / djj};
+        }
+        __syncthreads();
+        const int m = r - j - 1;
+        for (int e = tid; e < m * m; e += nthr) {
+            const int i = j + 1 + e / m, l = j + 1 + e % m;
+            if (l > i) continue;
+            const c64 a = A[(int64_t)i * ld + j], b = A[(int64_t)l * ld + j];
+            c64 &v = A[(int64_t)i * ld + l];
+            v = csub(v, cmul(a, cconj(b)));
+        }
+        __syncthreads();
+    }
+    if (tid == 0) status[k] = bad;
+    if (!bad)
+        for (int e = tid; e < r * r; e += nthr) {
+            const int i = e / r, j = e % r;
+            if (j > i) A[(int64_t)i * ld + j] = {0.0, 0.0};
+        }
+}
+
+// Linv = inv(L) for lower-triangular L; one thread per column.
+__global__ void __launch_bounds__(256) k_trinv_lower(c64 *__restrict__ Linv_all, const c64 *__restrict__ L_all,
+                                                    const int32_t *__restrict__ rk, int ld, int64_t stride) {
+    const int k = blockIdx.y;
+    const int r = rk[k];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    const c64 *L = L_all + k * stride;
+    c64 *X = Linv_all + k * stride;
+    for (int i = 0; i < c; ++i) X[(int64_t)i * ld + c] = {0.0, 0.0};
+    for (int i = c; i < r; ++i) {
+        c64 s = {i == c ? 1.0 : 0.0, 0.0};
+        for (int j = c; j < i; ++j) s = csub(s, cmul(L[(int64_t)i * ld + j], X[(int64_t)j * ld + c]));
+        const c64 d = L[(int64_t)i * ld + i];
+        const double dd = cabs2(d);
+        X[(int64_t)i * ld + c] = {(s.re * d.re + s.im * d.im) / dd, (s.im * d.re - s.re * d.im) / dd};
+    }
+}
+
+// C = op(A) op(B), op 0: identity, 1: conjugate transpose; r x r problems
+__global__ void k_small_gemm(c64 *__restrict__ C_all, const c64 *__restrict__ A_all, int opa,
+                             const c64 *__restrict__ B_all, int opb, const int32_t *__restrict__ rk, int ld,
+                             int64_t stride) {
+    const int k = blockIdx.z;
+    const int r = rk[k];
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r || j >= r) return;
+    const c64 *A = A_all + k * stride, *B = B_all + k * stride;
+    c64 s = {0.0, 0.0};
+    for (int l = 0; l < r; ++l) {
+        const c64 a = opa ? cconj(A[(int64_t)l * ld + i]) : A[(int64_t)i * ld + l];
+        const c64 b = opb ? cconj(B[(int64_t)j * ld + l]) : B[(int64_t)l * ld + j];
+        s = cadd(s, cmul(a, b));
+    }
+    C_all[k * stride + (int64_t)i * ld + j] = s;
+}
+
+// Out[k][i][j] = In[k][i][cols[k][j]] * scale[k][j] for i < rows, j < ncols[k]
+__global__ void k_gather_cols(c64 *__restrict__ Out, int ldo, int64_t o_stride, const c64 *__restrict__ In,
+                              int ldi, int64_t i_stride, const int32_t *__restrict__ cols,
+                              const double *__restrict__ scale, int cap, const int32_t *__restrict__ ncols,
+                              int rows) {
+    const int k = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = ncols[k];
+    if (e >= rows * cap) return;
+    const int i = e / cap, j = e % cap;
+    c64 v = {0.0, 0.0};
+    if (j < nc) {
+        const c64 x = In[k * i_stride + (int64_t)i * ldi + cols[k * cap + j]];
+        const double sc = scale[k * cap + j];
+        v = {x.re * sc, x.im * sc};
+    }
+    Out[k * o_stride + (int64_t)i * ldo + j] = v;
+}
+
+// Ghost-filter scores (ciss.py:153-177): score_j = sum_i |C_ij|^2 / sum_i |C_ij|^2 / sigma_i
+__global__ void k_ghost_scores(double *__restrict__ score, const c64 *__restrict__ C_all, int ld, int64_t stride,
+                               const double *__restrict__ sigma, int sig_ld, const int32_t *__restrict__ rank) {
+    const int k = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = rank[k];
+    if (j >= r) return;
+    const c64 *C = C_all + k * stride;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < r; ++i) {
+        const double a = cabs2(C[(int64_t)i * ld + j]);
+        num += a;
+        den += a / sigma[(int64_t)k * sig_ld + i];
+    }
+    score[(int64_t)k * sig_ld + j] = den > 0.0 ? num / den : 0.0;
+}
+
+// norm2[k][j] += || (H_k - e_kj S_k) v_kj ||^2 over a row range (FP64).
+// V: [k][n][cap] complex128; one wave per row, lanes over columns.
+__global__ void __launch_bounds__(256) k_eig_residual(double *__restrict__ norm2, const double2 *__restrict__ V,
+                                                     int cap, const int32_t *__restrict__ count,
+                                                     const double *__restrict__ energies,
+                                                     const float2 *__restrict__ h, const float2 *__restrict__ s,
+                                                     const int32_t *__restrict__ indptr,
+                                                     const int32_t *__restrict__ indices, int64_t n,
+                                                     int64_t nnz) {
+    const int k = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cnt = count[k];
+    const float2 *hk = h + (int64_t)k * nnz, *sk = s + (int64_t)k * nnz;
+    const double2 *vk = V + (int64_t)k * n * cap;
+    for (int c0 = 0; c0 < cnt; c0 += 64) {
+        const int c = c0 + lane;
+        double local = 0.0;
+        if (c < cnt) {
+            const double e = energies[(int64_t)k * cap + c];
+            for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < n; row += (int64_t)gridDim.x * 4) {
+                double rr = 0.0, ri = 0.0;
+                for (int32_t p = indptr[row]; p < indptr[row + 1]; ++p) {
+                    const double vr = (double)hk[p].x - e * sk[p].x;
+                    const double vi = (double)hk[p].y - e * sk[p].y;
+                    const double2 x = vk[(int64_t)indices[p] * cap + c];
+                    rr += vr * x.x - vi * x.y;
+                    ri += vr * x.y + vi * x.x;
+                }
+                local += rr * rr + ri * ri;
+            }
+            atomicAdd(norm2 + (int64_t)k * cap + c, local);
+        }
+    }
+}
+
+}  // namespace aosx

@@ -1,0 +1,767 @@
+// Host pipeline of the Band path on one GPU: resident bundle data, Bloch
+// assembly, batched tile LU of shifted matrices, triangular solves with
+// FP64-residual refinement, and the CISS subspace stage.
+//
+// The CISS steps follow src/aospectrum/ciss.py:266-391: contour geometry
+// (ciss.py:81-104), moments -w_j t_j^m X_j (ciss.py:324-333), rank-revealing
+// S-orthonormalisation (ciss.py:107-126), Rayleigh-Ritz (ciss.py:129-150), the
+// sigma-weighted ghost filter (ciss.py:153-177), the strict interval selection
+// and the maximum eigenpair residual (ciss.py:355-372).  Added here:
+//   * conjugate pairs: z_{N-1-j} = conj(z_j) and (H - conj(z) S) = (H - z S)^H,
+//     so one factorisation serves both shifts (the second through A^H = U^H L^H);
+//   * filter passes: the Ritz block of the previous pass is filtered again
+//     through the resident factorisations (moment 0 only) and projected anew,
+//     which squares the out-of-window leak of the contour filter.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "bundle.h"
+#include "kernels_asm.hip"
+#include "kernels_dense.hip"
+#include "kernels_lu.hip"
+#include "kernels_solve.hip"
+
+namespace aosx {
+
+namespace {
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+inline int blocks_for(int64_t work, int per, int cap) {
+    int64_t b = (work + per - 1) / per;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, cap));
+}
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on;
+    hipStream_t s;
+    EventTimer(bool enabled, hipStream_t st) : on(enabled), s(st) {
+        if (on) {
+            AOSX_HIP(hipEventCreate(&a));
+            AOSX_HIP(hipEventCreate(&b));
+            AOSX_HIP(hipEventRecord(a, s));
+        }
+    }
+    double stop() {
+        if (!on) return 0.0;
+        AOSX_HIP(hipEventRecord(b, s));
+        AOSX_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        AOSX_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
+Bundle::~Bundle() {
+    if (device >= 0) {
+        (void)hipSetDevice(device);
+        release_workspace();
+        for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
+                        (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
+                        (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev})
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+}
+
+void *Bundle::raw(const std::string &name, size_t bytes) {
+    Slot &s = ws[name];
+    if (s.bytes >= bytes && s.ptr) return s.ptr;
+    if (s.ptr) {
+        AOSX_HIP(hipStreamSynchronize(stream));
+        AOSX_HIP(hipFree(s.ptr));
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    const size_t grain = 2u << 20;
+    const size_t want = std::max<size_t>(grain, (bytes + grain - 1) / grain * grain);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(AOSX_MEMORY_ESTIMATE,
+                    "device allocation of " + std::to_string(want) + " bytes for '" + name + "' failed: " +
+                        hipGetErrorString(e),
+                    (int)e);
+    }
+    s.ptr = p;
+    s.bytes = want;
+    peak_bytes = std::max<uint64_t>(peak_bytes, resident_bytes + workspace_bytes());
+    return p;
+}
+
+void Bundle::release_workspace() {
+    for (auto &kv : ws)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    ws.clear();
+}
+
+uint64_t Bundle::workspace_bytes() const {
+    uint64_t t = 0;
+    for (auto &kv : ws) t += kv.second.bytes;
+    return t;
+}
+
+void Bundle::zero(void *p, size_t bytes) {
+    const int64_t c4 = (int64_t)(bytes / 16);
+    k_fill_zero<<<blocks_for(c4, 256, 8192), 256, 0, stream>>>(static_cast<float4 *>(p), c4);
+    AOSX_LAUNCHED();
+}
+
+template <typename T>
+static T *upload_vec(const std::vector<T> &v, uint64_t &counter) {
+    T *p = nullptr;
+    const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    AOSX_HIP(hipMalloc(&p, bytes));
+    if (!v.empty()) AOSX_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    counter += bytes;
+    return p;
+}
+
+void Bundle::upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values) {
+    AOSX_HIP(hipSetDevice(device));
+    AOSX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    std::vector<float2> tmp(n_values);
+    auto convert = [&](const void *src) {
+        if (value_kind == 0) {
+            const float *f = static_cast<const float *>(src);
+            for (int64_t i = 0; i < n_values; ++i) tmp[i] = make_float2(f[i], 0.f);
+        } else {
+            std::memcpy(tmp.data(), src, n_values * sizeof(float2));
+        }
+    };
+    convert(h_values);
+    d_hval = upload_vec(tmp, resident_bytes);
+    convert(s_values);
+    d_sval = upload_vec(tmp, resident_bytes);
+    std::vector<int32_t> src32(pat.source.size()), starts32(pat.starts.size());
+    for (size_t i = 0; i < src32.size(); ++i) src32[i] = (int32_t)pat.source[i];
+    for (size_t i = 0; i < starts32.size(); ++i) starts32[i] = (int32_t)pat.starts[i];
+    d_source = upload_vec(src32, resident_bytes);
+    d_starts = upload_vec(starts32, resident_bytes);
+    d_group = upload_vec(pat.group, resident_bytes);
+    d_mirror = upload_vec(pat.mirror, resident_bytes);
+    d_rows = upload_vec(pat.rows, resident_bytes);
+    d_indices = upload_vec(pat.indices, resident_bytes);
+    d_indptr = upload_vec(pat.indptr, resident_bytes);
+    d_shifts = upload_vec(pat.shifts, resident_bytes);
+    d_iperm = upload_vec(ts.iperm, resident_bytes);
+    d_perm = upload_vec(ts.perm, resident_bytes);
+    d_tile_id = upload_vec(ts.tile_id, resident_bytes);
+    d_next = upload_vec(ts.next_idx, resident_bytes);
+    d_prev = upload_vec(ts.prev_idx, resident_bytes);
+    peak_bytes = resident_bytes;
+}
+
+// ------------------------------------------------------------------ assembly
+void Bundle::assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *d_s) {
+    double *d_k = buf<double>("kpts", 3 * nk);
+    AOSX_HIP(hipMemcpyAsync(d_k, kpts_host, 3 * nk * sizeof(double), hipMemcpyHostToDevice, stream));
+    float2 *d_ph = buf<float2>("phase", n_groups * nk);
+    const int ng = (int)n_groups;
+    k_phases<<<blocks_for(ng * nk, 256, 1 << 20), 256, 0, stream>>>(d_ph, d_shifts, ng, d_k, (int)nk);
+    AOSX_LAUNCHED();
+    EventTimer tm(kstats.enabled, stream);
+    dim3 grid(blocks_for(nnz, 256, 4096), (unsigned)nk);
+    k_assemble<<<grid, 256, 0, stream>>>(d_h, d_s, d_hval, d_sval, d_source, d_group, d_starts, d_mirror, d_ph,
+                                         ng, nnz);
+    AOSX_LAUNCHED();
+    if (kstats.enabled) {
+        kstats.assemble_ms += tm.stop();
+        kstats.assemble_launches += 1;
+        // algorithmic bytes: each CSR entry gathers its own and its mirror's
+        // source values of H and S and writes two complex64 values
+        const double src = (double)pat.source.size();
+        kstats.assemble_bytes += nk * (2.0 * 2.0 * src * (8.0 + 4.0 + 4.0) + nnz * (2 * 8.0 + 4 * 4.0 + 4.0));
+    }
+}
+
+// ------------------------------------------------------------- factorization
+void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
+                    float *tiles, float *dinv, int *d_replaced) {
+    // the H, S values of the current k batch live in the "H"/"S" workspaces
+    const float2 *d_h = static_cast<const float2 *>(ws.at("H").ptr);
+    const float2 *d_s = static_cast<const float2 *>(ws.at("S").ptr);
+    int32_t *d_kidx = buf<int32_t>("f_kidx", nmat);
+    float2 *d_z = buf<float2>("f_z", nmat);
+    AOSX_HIP(hipMemcpyAsync(d_kidx, kidx.data(), nmat * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_z, z.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
+    const int64_t mstride = (int64_t)factor_floats();
+    const int64_t count4 = (int64_t)nmat * mstride / 4;
+    k_fill_zero<<<blocks_for(count4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(tiles), count4);
+    AOSX_LAUNCHED();
+    k_scatter_shifted<<<dim3(blocks_for(nnz, 256, 4096), nmat), 256, 0, stream>>>(
+        tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, n);
+    AOSX_LAUNCHED();
+    const int nt = ts.n_diag;
+    const float pivot_rel = 1.0e-6f;
+    for (int k = 0; k < nt; ++k) {
+        const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+        const int32_t *nx = d_next + ts.next_ptr[k];
+        {
+            EventTimer tm(kstats.enabled, stream);
+            k_getrf_diag<<<nmat, 256, 0, stream>>>(tiles, mstride, dinv, nt, k, d_tile_id, d_replaced, pivot_rel);
+            AOSX_LAUNCHED();
+            if (kstats.enabled) { kstats.getrf_ms += tm.stop(); kstats.getrf_launches++; }
+        }
+        if (s == 0) continue;
+        {
+            EventTimer tm(kstats.enabled, stream);
+            k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id);
+            AOSX_LAUNCHED();
+            if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
+        }
+        {
+            EventTimer tm(kstats.enabled, stream);
+            dim3 grid(s * s, nmat);
+            if (precision == AOSX_PRECISION_BF16X3_IR)
+                k_schur<1><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
+            else if (precision == AOSX_PRECISION_BF16_IR)
+                k_schur<2><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
+            else
+                k_schur<0><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
+            AOSX_LAUNCHED();
+            if (kstats.enabled) {
+                kstats.schur_ms += tm.stop();
+                kstats.schur_launches++;
+                kstats.schur_flops += 8.0 * 64.0 * 64.0 * 64.0 * (double)s * s * nmat;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- solves
+void Bundle::solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans) {
+    const int nt = ts.n_diag, chunks = W / 32;
+    const int64_t ystride = (int64_t)nt * 64 * W;
+    const int64_t mstride = (int64_t)factor_floats();
+    EventTimer tm(kstats.enabled, stream);
+    for (int k = 0; k < nt; ++k) {
+        k_solve_diag<<<dim3(chunks, nmat), 256, 0, stream>>>(y, ystride, W, dinv, nt, k, ctrans ? 1 : 0,
+                                                             ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+        const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+        if (s) {
+            k_solve_update<<<dim3(chunks, s, nmat), 256, 0, stream>>>(y, ystride, W, tiles, mstride, d_tile_id,
+                                                                      nt, k, d_next + ts.next_ptr[k],
+                                                                      ctrans ? 1 : 0);
+            AOSX_LAUNCHED();
+        }
+    }
+    for (int k = nt - 1; k >= 0; --k) {
+        k_solve_diag<<<dim3(chunks, nmat), 256, 0, stream>>>(y, ystride, W, dinv, nt, k, ctrans ? 0 : 1,
+                                                             ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+        const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
+        if (s) {
+            k_solve_update<<<dim3(chunks, s, nmat), 256, 0, stream>>>(y, ystride, W, tiles, mstride, d_tile_id,
+                                                                      nt, k, d_prev + ts.prev_ptr[k],
+                                                                      ctrans ? 1 : 0);
+            AOSX_LAUNCHED();
+        }
+    }
+    if (kstats.enabled) {
+        kstats.solve_ms += tm.stop();
+        kstats.solve_launches += 1;
+        // every stored tile is read once per solve, every RHS tile twice
+        kstats.solve_bytes += (double)nmat * ((double)ts.n_tiles * 2 * 4 * 4096 + 4.0 * nt * 64 * W * 8);
+    }
+}
+
+void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
+                        const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
+                        const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
+                        float2 *x_out, double *rel_residual) {
+    const int nt = ts.n_diag;
+    const int64_t padded = (int64_t)nt * 64;
+    const int64_t ystride = padded * W;
+    float2 *y = buf<float2>("s_y", (size_t)nmat * ystride);
+    int32_t *d_ri = buf<int32_t>("s_rhs_index", nmat);
+    int32_t *d_id = buf<int32_t>("s_identity", nmat);
+    int32_t *d_ki = buf<int32_t>("s_kidx", nmat);
+    float2 *d_z = buf<float2>("s_z", nmat);
+    std::vector<int32_t> ident(nmat);
+    for (int i = 0; i < nmat; ++i) ident[i] = i;
+    AOSX_HIP(hipMemcpyAsync(d_ri, rhs_index.data(), nmat * 4, hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_id, ident.data(), nmat * 4, hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_ki, kidx.data(), nmat * 4, hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_z, zeff.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
+    const int pb = blocks_for(padded * W, 256, 4096);
+    k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, W, d_rhs, W, d_ri, d_perm, n, padded);
+    AOSX_LAUNCHED();
+    solve(nmat, W, y, tiles, dinv, ctrans);
+    k_unpack<<<dim3(blocks_for(n * W, 256, 4096), nmat), 256, 0, stream>>>(x_out, W, y, ystride, W, d_perm, n);
+    AOSX_LAUNCHED();
+    if (ir_steps <= 0 && !rel_residual) return;
+    float2 *r = buf<float2>("s_r", (size_t)nmat * n * W);
+    double *d_norm = buf<double>("s_norm", 2 * (size_t)nmat);
+    const int rb = blocks_for(n, 4, 2048);
+    for (int step = 0; step <= ir_steps; ++step) {
+        const bool last = step == ir_steps;
+        if (last && !rel_residual) break;
+        k_fill_zero<<<1, 64, 0, stream>>>(reinterpret_cast<float4 *>(d_norm), (int64_t)nmat * 2 * 8 / 16);
+        AOSX_LAUNCHED();
+        k_residual_shifted<<<dim3(rb, nmat), 256, 0, stream>>>(r, d_rhs, d_ri, x_out, W, d_h, d_s, d_ki, d_z,
+                                                               d_indptr, d_indices, n, nnz, d_norm, d_norm + nmat);
+        AOSX_LAUNCHED();
+        if (last) {
+            std::vector<double> hn(2 * (size_t)nmat);
+            AOSX_HIP(hipMemcpyAsync(hn.data(), d_norm, hn.size() * 8, hipMemcpyDeviceToHost, stream));
+            AOSX_HIP(hipStreamSynchronize(stream));
+            double worst = 0.0;
+            for (int i = 0; i < nmat; ++i)
+                worst = std::max(worst, std::sqrt(hn[i] / std::max(hn[nmat + i], 1e-300)));
+            *rel_residual = worst;
+            break;
+        }
+        k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, W, r, W, d_id, d_perm, n, padded);
+        AOSX_LAUNCHED();
+        solve(nmat, W, y, tiles, dinv, ctrans);
+        k_unpack<<<dim3(blocks_for(n * W, 256, 4096), nmat), 256, 0, stream>>>(r, W, y, ystride, W, d_perm, n);
+        AOSX_LAUNCHED();
+        k_axpy<<<blocks_for((int64_t)nmat * n * W, 256, 8192), 256, 0, stream>>>(x_out, r, (int64_t)nmat * n * W);
+        AOSX_LAUNCHED();
+    }
+}
+
+// ---------------------------------------------------------------- CISS band
+namespace {
+struct Subspace {
+    int rmax = 0;                 // padded rank width (multiple of 32)
+    std::vector<int> rank;        // per k
+    std::vector<std::vector<float>> energies;
+    std::vector<double> residual;
+    float2 *ritz = nullptr;       // [nk][n][rmax] Ritz block Q C (when requested)
+};
+}  // namespace
+
+static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, const float2 *d_H,
+                                 const float2 *d_S, const CissParams &p, Subspace &out, double *stage,
+                                 const char *ritz_name) {
+    const int64_t n = B.n, nnz = B.nnz;
+    hipStream_t st = B.stream;
+    double t0 = now();
+    std::vector<int32_t> iota_k(nk);
+    for (int k = 0; k < nk; ++k) iota_k[k] = k;
+    int32_t *d_iota = B.buf<int32_t>("c_iota", nk);
+    AOSX_HIP(hipMemcpyAsync(d_iota, iota_k.data(), nk * 4, hipMemcpyHostToDevice, st));
+    // SM = S M, G = M^H S M (FP64, hermitised)
+    float2 *d_SM = B.buf<float2>("c_SM", (size_t)nk * n * P);
+    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_SM, d_S, d_iota, d_M, d_iota, P, B.d_indptr,
+                                                             B.d_indices, n, nnz);
+    AOSX_LAUNCHED();
+    auto gram = [&](c64 *G, int ldg, const float2 *A, const float2 *Bm, int Pc, int64_t stride_a, int herm) {
+        const int chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, n / 256));
+        const int rpc = (int)(((n + chunks - 1) / chunks + 31) / 32 * 32);
+        c64 *part = B.buf<c64>("c_part", (size_t)nk * chunks * Pc * Pc);
+        const int tiles = ((Pc + 31) / 32) * ((Pc + 31) / 32);
+        k_gram_partial<<<dim3(tiles, chunks, nk), 256, 0, st>>>(part, A, stride_a, Pc, Bm, stride_a, Pc, n, rpc);
+        AOSX_LAUNCHED();
+        k_gram_reduce<<<dim3(blocks_for((int64_t)Pc * Pc, 256, 1024), nk), 256, 0, st>>>(
+            G, ldg, (int64_t)ldg * ldg, part, Pc, Pc, chunks, herm);
+        AOSX_LAUNCHED();
+    };
+    c64 *d_G = B.buf<c64>("c_G", (size_t)nk * P * P);
+    gram(d_G, P, d_M, d_SM, P, (int64_t)n * P, 1);
+    // eigh(G)
+    c64 *d_V = B.buf<c64>("c_V", (size_t)nk * P * P);
+    c64 *d_U = B.buf<c64>("c_U", (size_t)nk * P * P);
+    double *d_w = B.buf<double>("c_w", (size_t)nk * P);
+    std::vector<int32_t> rP(nk, P);
+    int32_t *d_rP = B.buf<int32_t>("c_rP", nk);
+    AOSX_HIP(hipMemcpyAsync(d_rP, rP.data(), nk * 4, hipMemcpyHostToDevice, st));
+    k_jacobi<<<nk, 1024, 0, st>>>(d_G, d_V, d_U, d_w, d_rP, P, (int64_t)P * P, 40, 1e-15, nullptr);
+    AOSX_LAUNCHED();
+    std::vector<double> w((size_t)nk * P);
+    AOSX_HIP(hipMemcpyAsync(w.data(), d_w, w.size() * 8, hipMemcpyDeviceToHost, st));
+    AOSX_HIP(hipStreamSynchronize(st));
+    // rank truncation (ciss.py:107-126)
+    out.rank.assign(nk, 0);
+    std::vector<std::vector<int32_t>> keep(nk);
+    std::vector<std::vector<double>> sig(nk);
+    int rmax = 0;
+    for (int k = 0; k < nk; ++k) {
+        double scale = 0.0;
+        for (int i = 0; i < P; ++i) scale = std::max(scale, std::fabs(w[(size_t)k * P + i]));
+        if (scale == 0.0 || !std::isfinite(scale)) throw Error(AOSX_ZERO_RANK, "CISS contour moments have zero numerical rank");
+        for (int i = 0; i < P; ++i)
+            if (w[(size_t)k * P + i] > scale * p.rank_tol) {
+                keep[k].push_back(i);
+                sig[k].push_back(std::sqrt(w[(size_t)k * P + i]));
+            }
+        if (keep[k].empty()) throw Error(AOSX_NO_STABLE_SUBSPACE, "CISS contour moments have no positive stable subspace");
+        out.rank[k] = (int)keep[k].size();
+        rmax = std::max(rmax, out.rank[k]);
+    }
+    const int R = (rmax + 31) / 32 * 32;
+    out.rmax = R;
+    std::vector<int32_t> cols((size_t)nk * R, 0), rk(nk);
+    std::vector<double> scl((size_t)nk * R, 0.0), sigma((size_t)nk * R, 1.0);
+    for (int k = 0; k < nk; ++k) {
+        rk[k] = out.rank[k];
+        for (int j = 0; j < out.rank[k]; ++j) {
+            cols[(size_t)k * R + j] = keep[k][j];
+            scl[(size_t)k * R + j] = 1.0 / sig[k][j];
+            sigma[(size_t)k * R + j] = sig[k][j];
+        }
+    }
+    int32_t *d_cols = B.buf<int32_t>("c_cols", (size_t)nk * R);
+    double *d_scl = B.buf<double>("c_scl", (size_t)nk * R);
+    double *d_sigma = B.buf<double>("c_sigma", (size_t)nk * R);
+    int32_t *d_rk = B.buf<int32_t>("c_rk", nk);
+    AOSX_HIP(hipMemcpyAsync(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice, st));
+    AOSX_HIP(hipMemcpyAsync(d_scl, scl.data(), scl.size() * 8, hipMemcpyHostToDevice, st));
+    AOSX_HIP(hipMemcpyAsync(d_sigma, sigma.data(), sigma.size() * 8, hipMemcpyHostToDevice, st));
+    AOSX_HIP(hipMemcpyAsync(d_rk, rk.data(), nk * 4, hipMemcpyHostToDevice, st));
+    // T = U[:, keep] diag(lambda^-1/2)   (P x R), Q = M T (n x R, complex64)
+    c64 *d_T = B.buf<c64>("c_T", (size_t)nk * P * R);
+    k_gather_cols<<<dim3(blocks_for((int64_t)P * R, 256, 4096), nk), 256, 0, st>>>(
+        d_T, R, (int64_t)P * R, d_U, P, (int64_t)P * P, d_cols, d_scl, R, d_rk, P);
+    AOSX_LAUNCHED();
+    float2 *d_Q = B.buf<float2>("c_Q", (size_t)nk * n * R);
+    k_tall_gemm<float2><<<dim3((unsigned)((n + 7) / 8), R / 32, nk), 256, 0, st>>>(
+        d_Q, (int64_t)n * R, d_M, (int64_t)n * P, P, d_T, R, (int64_t)P * R, R, n);
+    AOSX_LAUNCHED();
+    AOSX_HIP(hipStreamSynchronize(st));
+    double t1 = now();
+    stage[6] += t1 - t0;
+    // Rayleigh-Ritz: Hh = Q^H H Q, Sh = Q^H S Q (FP64, hermitised)
+    float2 *d_HQ = B.buf<float2>("c_HQ", (size_t)nk * n * R);
+    float2 *d_SQ = B.buf<float2>("c_SQ", (size_t)nk * n * R);
+    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_HQ, d_H, d_iota, d_Q, d_iota, R, B.d_indptr,
+                                                             B.d_indices, n, nnz);
+    AOSX_LAUNCHED();
+    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_SQ, d_S, d_iota, d_Q, d_iota, R, B.d_indptr,
+                                                             B.d_indices, n, nnz);
+    AOSX_LAUNCHED();
+    const int64_t RR = (int64_t)R * R;
+    c64 *d_Hh = B.buf<c64>("c_Hh", (size_t)nk * RR);
+    c64 *d_Sh = B.buf<c64>("c_Sh", (size_t)nk * RR);
+    gram(d_Hh, R, d_Q, d_HQ, R, (int64_t)n * R, 1);
+    gram(d_Sh, R, d_Q, d_SQ, R, (int64_t)n * R, 1);
+    // Cholesky Sh = L L^H, W = L^-1 Hh L^-H, eigh(W), C = L^-H Y
+    int32_t *d_status = B.buf<int32_t>("c_status", nk);
+    k_cholesky<<<nk, 1024, 0, st>>>(d_Sh, d_rk, R, RR, d_status);
+    AOSX_LAUNCHED();
+    std::vector<int32_t> status(nk);
+    AOSX_HIP(hipMemcpyAsync(status.data(), d_status, nk * 4, hipMemcpyDeviceToHost, st));
+    AOSX_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nk; ++k)
+        if (status[k]) throw Error(AOSX_OVERLAP_NOT_PD, "CISS reduced overlap is not positive definite");
+    c64 *d_Li = B.buf<c64>("c_Li", (size_t)nk * RR);
+    c64 *d_Tm = B.buf<c64>("c_Tm", (size_t)nk * RR);
+    c64 *d_Wm = B.buf<c64>("c_Wm", (size_t)nk * RR);
+    c64 *d_C = B.buf<c64>("c_C", (size_t)nk * RR);
+    k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Li), nk * RR);
+    k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Tm), nk * RR);
+    k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Wm), nk * RR);
+    k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_C), nk * RR);
+    AOSX_LAUNCHED();
+    k_trinv_lower<<<dim3((R + 255) / 256, nk), 256, 0, st>>>(d_Li, d_Sh, d_rk, R, RR);
+    AOSX_LAUNCHED();
+    const dim3 sg((R + 63) / 64, R, nk);
+    k_small_gemm<<<sg, 64, 0, st>>>(d_Tm, d_Li, 0, d_Hh, 0, d_rk, R, RR);
+    k_small_gemm<<<sg, 64, 0, st>>>(d_Wm, d_Tm, 0, d_Li, 1, d_rk, R, RR);
+    AOSX_LAUNCHED();
+    c64 *d_Y = B.buf<c64>("c_Y", (size_t)nk * RR);
+    c64 *d_Yw = B.buf<c64>("c_Yw", (size_t)nk * RR);
+    double *d_E = B.buf<double>("c_E", (size_t)nk * R);
+    k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Y), nk * RR);
+    AOSX_LAUNCHED();
+    k_jacobi<<<nk, 1024, 0, st>>>(d_Wm, d_Yw, d_Y, d_E, d_rk, R, RR, 40, 1e-15, nullptr);
+    AOSX_LAUNCHED();
+    k_small_gemm<<<sg, 64, 0, st>>>(d_C, d_Li, 1, d_Y, 0, d_rk, R, RR);
+    AOSX_LAUNCHED();
+    double *d_score = B.buf<double>("c_score", (size_t)nk * R);
+    k_ghost_scores<<<dim3((R + 63) / 64, nk), 64, 0, st>>>(d_score, d_C, R, RR, d_sigma, R, d_rk);
+    AOSX_LAUNCHED();
+    std::vector<double> E((size_t)nk * R), score((size_t)nk * R);
+    AOSX_HIP(hipMemcpyAsync(E.data(), d_E, E.size() * 8, hipMemcpyDeviceToHost, st));
+    AOSX_HIP(hipMemcpyAsync(score.data(), d_score, score.size() * 8, hipMemcpyDeviceToHost, st));
+    AOSX_HIP(hipStreamSynchronize(st));
+    if (ritz_name) {
+        // all Ritz vectors Q C (complex64, n x R), zero beyond each k's rank
+        out.ritz = B.buf<float2>(ritz_name, (size_t)nk * n * R);
+        k_tall_gemm<float2><<<dim3((unsigned)((n + 7) / 8), R / 32, nk), 256, 0, st>>>(
+            out.ritz, (int64_t)n * R, d_Q, (int64_t)n * R, R, d_C, R, RR, R, n);
+        AOSX_LAUNCHED();
+    }
+    double t2 = now();
+    stage[7] += t2 - t1;
+    // ghost filter and strict interval selection (ciss.py:350-361)
+    std::vector<std::vector<int32_t>> sel(nk);
+    out.energies.assign(nk, {});
+    int mmax = 0;
+    for (int k = 0; k < nk; ++k) {
+        double top = 0.0;
+        for (int j = 0; j < out.rank[k]; ++j) top = std::max(top, score[(size_t)k * R + j]);
+        if (top == 0.0 || !std::isfinite(top))
+            throw Error(AOSX_GHOST_FILTER, "CISS ghost filter produced no positive finite score");
+        for (int j = 0; j < out.rank[k]; ++j) {
+            const double sc = score[(size_t)k * R + j];
+            const float e32 = (float)E[(size_t)k * R + j];
+            if (std::isfinite(sc) && sc >= p.spurious * top && (double)e32 > p.lower && (double)e32 < p.upper) {
+                sel[k].push_back(j);
+                out.energies[k].push_back(e32);
+            }
+        }
+        mmax = std::max<int>(mmax, (int)sel[k].size());
+    }
+    out.residual.assign(nk, 0.0);
+    if (mmax > 0) {
+        const int Mc = mmax;
+        std::vector<int32_t> scols((size_t)nk * Mc, 0), scnt(nk);
+        std::vector<double> sone((size_t)nk * Mc, 1.0), senergy((size_t)nk * Mc, 0.0);
+        for (int k = 0; k < nk; ++k) {
+            scnt[k] = (int32_t)sel[k].size();
+            for (size_t j = 0; j < sel[k].size(); ++j) {
+                scols[(size_t)k * Mc + j] = sel[k][j];
+                senergy[(size_t)k * Mc + j] = (double)out.energies[k][j];
+            }
+        }
+        int32_t *d_scols = B.buf<int32_t>("r_cols", (size_t)nk * Mc);
+        int32_t *d_scnt = B.buf<int32_t>("r_cnt", nk);
+        double *d_sone = B.buf<double>("r_one", (size_t)nk * Mc);
+        double *d_sen = B.buf<double>("r_energy", (size_t)nk * Mc);
+        AOSX_HIP(hipMemcpyAsync(d_scols, scols.data(), scols.size() * 4, hipMemcpyHostToDevice, st));
+        AOSX_HIP(hipMemcpyAsync(d_scnt, scnt.data(), nk * 4, hipMemcpyHostToDevice, st));
+        AOSX_HIP(hipMemcpyAsync(d_sone, sone.data(), sone.size() * 8, hipMemcpyHostToDevice, st));
+        AOSX_HIP(hipMemcpyAsync(d_sen, senergy.data(), senergy.size() * 8, hipMemcpyHostToDevice, st));
+        c64 *d_Cs = B.buf<c64>("r_Cs", (size_t)nk * R * Mc);
+        k_gather_cols<<<dim3(blocks_for((int64_t)R * Mc, 256, 4096), nk), 256, 0, st>>>(
+            d_Cs, Mc, (int64_t)R * Mc, d_C, R, RR, d_scols, d_sone, Mc, d_scnt, R);
+        AOSX_LAUNCHED();
+        double2 *d_vec = B.buf<double2>("r_vec", (size_t)nk * n * Mc);
+        k_tall_gemm<double2><<<dim3((unsigned)((n + 7) / 8), (Mc + 31) / 32, nk), 256, 0, st>>>(
+            d_vec, (int64_t)n * Mc, d_Q, (int64_t)n * R, R, d_Cs, Mc, (int64_t)R * Mc, Mc, n);
+        AOSX_LAUNCHED();
+        double *d_n2 = B.buf<double>("r_norm2", (size_t)nk * Mc + 2);
+        const int64_t z4 = ((int64_t)nk * Mc * 8 + 15) / 16;
+        k_fill_zero<<<blocks_for(z4, 256, 1024), 256, 0, st>>>(reinterpret_cast<float4 *>(d_n2), z4);
+        k_eig_residual<<<dim3(blocks_for(n, 4, 1024), nk), 256, 0, st>>>(d_n2, d_vec, Mc, d_scnt, d_sen, d_H, d_S,
+                                                                       B.d_indptr, B.d_indices, n, nnz);
+        AOSX_LAUNCHED();
+        std::vector<double> n2((size_t)nk * Mc);
+        AOSX_HIP(hipMemcpyAsync(n2.data(), d_n2, n2.size() * 8, hipMemcpyDeviceToHost, st));
+        AOSX_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < scnt[k]; ++j)
+                out.residual[k] = std::max(out.residual[k], std::sqrt(n2[(size_t)k * Mc + j]));
+    }
+    stage[8] += now() - t2;
+}
+
+void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
+                  const float2 *probes, CissOutput &out) {
+    AOSX_HIP(hipSetDevice(device));
+    const double t_start = now();
+    const int N = p.integration_points, L = p.block_size, Ms = p.moment_size, P = L * Ms;
+    if (p.conjugate_pairs && (N % 2)) throw Error(AOSX_INVALID_SETTINGS, "conjugate pairs need an even integration_points");
+    if (L % 32) throw Error(AOSX_INVALID_SETTINGS, "block_size must be a multiple of 32 on this device path");
+    // contour (ciss.py:81-104), FP64 then complex64
+    const double lo = p.lower - 2.0 * p.guard, hi = p.upper + 2.0 * p.guard;
+    const double cen = 0.5 * (lo + hi), rh = 0.5 * (hi - lo), rv = rh * p.vscale;
+    std::vector<std::complex<double>> zc(N), tc(N), wc(N);
+    for (int j = 0; j < N; ++j) {
+        const double th = 2.0 * M_PI * (j + 0.5) / N;
+        tc[j] = {std::cos(th), p.vscale * std::sin(th)};
+        zc[j] = cen + rh * tc[j];
+        wc[j] = std::complex<double>(rv * std::cos(th), rh * std::sin(th)) / (double)N;
+    }
+    if (p.conjugate_pairs)
+        for (int j = N / 2; j < N; ++j) {
+            zc[j] = std::conj(zc[N - 1 - j]);
+            tc[j] = std::conj(tc[N - 1 - j]);
+            wc[j] = std::conj(wc[N - 1 - j]);
+        }
+    auto f2 = [](std::complex<double> v) { return make_float2((float)v.real(), (float)v.imag()); };
+    std::vector<std::complex<float>> t32(N), w32(N);
+    for (int j = 0; j < N; ++j) {
+        t32[j] = {(float)tc[j].real(), (float)tc[j].imag()};
+        w32[j] = {(float)wc[j].real(), (float)wc[j].imag()};
+    }
+    // moment coefficient of shift j, moment m: -w_j t_j^m in complex64 arithmetic
+    auto coef = [&](int j, int m) {
+        std::complex<float> tp(1.f, 0.f);
+        for (int q = 0; q < m; ++q) tp *= t32[j];
+        const std::complex<float> c = -w32[j] * tp;
+        return make_float2(c.real(), c.imag());
+    };
+    float2 *d_V = buf<float2>("probes", (size_t)n * L);
+    AOSX_HIP(hipMemcpyAsync(d_V, probes, (size_t)n * L * sizeof(float2), hipMemcpyHostToDevice, stream));
+    const int64_t nk_total = p.nk;
+    out.energies.assign(nk_total, {});
+    out.max_residual.assign(nk_total, 0.0);
+    out.rank.assign(nk_total, 0);
+    out.saturated.assign(nk_total, 0);
+    const int K = std::max(1, p.k_batch);
+    const int nfac = p.conjugate_pairs ? N / 2 : N;
+    const int cb = std::max(1, std::min(p.contour_batch_size, nfac));
+    const bool resident = p.refinement_passes > 0;
+    const bool ir = p.precision != AOSX_PRECISION_FP32;
+    const int ir_steps = ir ? std::max(0, p.refinement_steps) : 0;
+    int *d_repl = buf<int>("replaced", 65536);
+    out.stages[1] += now() - t_start;
+    for (int64_t kb = 0; kb < nk_total; kb += K) {
+        const int Kc = (int)std::min<int64_t>(K, nk_total - kb);
+        double t0 = now();
+        float2 *d_H = buf<float2>("H", (size_t)Kc * nnz);
+        float2 *d_S = buf<float2>("S", (size_t)Kc * nnz);
+        if (h_in) {
+            AOSX_HIP(hipMemcpyAsync(d_H, h_in + kb * nnz, (size_t)Kc * nnz * 8, hipMemcpyHostToDevice, stream));
+            AOSX_HIP(hipMemcpyAsync(d_S, s_in + kb * nnz, (size_t)Kc * nnz * 8, hipMemcpyHostToDevice, stream));
+        } else {
+            assemble(kpts + 3 * kb, Kc, d_H, d_S);
+        }
+        AOSX_HIP(hipStreamSynchronize(stream));
+        double t1 = now();
+        out.stages[0] += t1 - t0;
+        // RHS = S V per k
+        std::vector<int32_t> iota(Kc), zeros(Kc, 0);
+        for (int k = 0; k < Kc; ++k) iota[k] = k;
+        int32_t *d_iota = buf<int32_t>("b_iota", Kc), *d_zero = buf<int32_t>("b_zero", Kc);
+        AOSX_HIP(hipMemcpyAsync(d_iota, iota.data(), Kc * 4, hipMemcpyHostToDevice, stream));
+        AOSX_HIP(hipMemcpyAsync(d_zero, zeros.data(), Kc * 4, hipMemcpyHostToDevice, stream));
+        float2 *d_rhs = buf<float2>("b_rhs", (size_t)Kc * n * L);
+        k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs, d_S, d_iota, d_V, d_zero, L, d_indptr,
+                                                                    d_indices, n, nnz);
+        AOSX_LAUNCHED();
+        float2 *d_M = buf<float2>("b_M", (size_t)Kc * n * P);
+        const int64_t m4 = (int64_t)Kc * n * P * 8 / 16;
+        k_fill_zero<<<blocks_for(m4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(d_M), m4);
+        AOSX_LAUNCHED();
+        // factorisation groups: all k of the batch x cb shifts
+        const int G = Kc * cb;
+        const int ngroups = (nfac + cb - 1) / cb;
+        const int slots = resident ? Kc * nfac : G;
+        const size_t ff = factor_floats();
+        const size_t dif = (size_t)ts.n_diag * 2 * 2 * 64 * 64;
+        float *d_tiles = buf<float>("tiles", (size_t)slots * ff);
+        float *d_dinv = buf<float>("dinv", (size_t)slots * dif);
+        struct Group {
+            std::vector<int32_t> kidx;
+            std::vector<int> shift;
+            std::vector<float2> z, zc;
+            int slot0;
+        };
+        std::vector<Group> groups(ngroups);
+        for (int g = 0; g < ngroups; ++g) {
+            Group &gr = groups[g];
+            const int j0 = g * cb, j1 = std::min(nfac, j0 + cb);
+            for (int k = 0; k < Kc; ++k)
+                for (int j = j0; j < j1; ++j) {
+                    gr.kidx.push_back(k);
+                    gr.shift.push_back(j);
+                    gr.z.push_back(f2(zc[j]));
+                    gr.zc.push_back(f2(std::conj(zc[j])));
+                }
+            gr.slot0 = resident ? g * G : 0;
+        }
+        float2 *d_X = buf<float2>("b_X", (size_t)G * n * std::max(L, 32));
+        float2 *d_coef = buf<float2>("b_coef", (size_t)G * Ms);
+        auto accumulate = [&](const Group &gr, bool conj_side, float2 *dst, int width, int msize) {
+            // dst[k] += sum over the group's shifts of k of coef * X
+            const int per_k = (int)gr.kidx.size() / Kc;
+            std::vector<float2> cf(gr.kidx.size() * msize);
+            for (size_t b = 0; b < gr.kidx.size(); ++b)
+                for (int m = 0; m < msize; ++m) {
+                    float2 c = coef(gr.shift[b], m);
+                    if (conj_side) {
+                        const int jj = N - 1 - gr.shift[b];
+                        c = coef(jj, m);
+                    }
+                    cf[b * msize + m] = c;
+                }
+            AOSX_HIP(hipMemcpyAsync(d_coef, cf.data(), cf.size() * sizeof(float2), hipMemcpyHostToDevice, stream));
+            for (int k = 0; k < Kc; ++k) {
+                k_moments<<<blocks_for(n * width, 256, 4096), 256, 0, stream>>>(
+                    dst + (size_t)k * n * width * msize, d_X + (size_t)k * per_k * n * width, per_k, width, msize,
+                    d_coef + (size_t)k * per_k * msize, n);
+                AOSX_LAUNCHED();
+            }
+        };
+        const int64_t repl_zero4 = 65536 * 4 / 16;
+        k_fill_zero<<<64, 256, 0, stream>>>(reinterpret_cast<float4 *>(d_repl), repl_zero4);
+        AOSX_LAUNCHED();
+        for (int g = 0; g < ngroups; ++g) {
+            Group &gr = groups[g];
+            const int gsz = (int)gr.kidx.size();
+            float *tl = d_tiles + (size_t)gr.slot0 * ff;
+            float *di = d_dinv + (size_t)gr.slot0 * dif;
+            double ta = now();
+            factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0);
+            AOSX_HIP(hipStreamSynchronize(stream));
+            double tb = now();
+            out.stages[2] += tb - ta;
+            out.factorizations += gsz;
+            solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_steps, d_X, nullptr);
+            accumulate(gr, false, d_M, L, Ms);
+            if (p.conjugate_pairs) {
+                solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_steps, d_X, nullptr);
+                accumulate(gr, true, d_M, L, Ms);
+            }
+            AOSX_HIP(hipStreamSynchronize(stream));
+            out.stages[3] += now() - tb;
+        }
+        std::vector<int> repl(slots);
+        AOSX_HIP(hipMemcpy(repl.data(), d_repl, slots * sizeof(int), hipMemcpyDeviceToHost));
+        for (int v : repl) out.replaced_pivots += v;
+        // subspace stage
+        Subspace sub;
+        const char *ritz_names[2] = {"b_ritz0", "b_ritz1"};
+        compress_and_project(*this, Kc, d_M, P, d_H, d_S, p, sub, out.stages,
+                             p.refinement_passes > 0 ? ritz_names[0] : nullptr);
+        std::vector<int> rank0 = sub.rank;
+        for (int pass = 0; pass < p.refinement_passes; ++pass) {
+            double tp = now();
+            const int W2 = sub.rmax;  // multiple of 32
+            // RHS2 = S * Ritz block of the previous projection
+            float2 *d_rhs2 = buf<float2>("p_rhs", (size_t)Kc * n * W2);
+            k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs2, d_S, d_iota, sub.ritz, d_iota, W2,
+                                                                        d_indptr, d_indices, n, nnz);
+            AOSX_LAUNCHED();
+            float2 *d_Y = buf<float2>("p_Y", (size_t)Kc * n * W2);
+            const int64_t y4 = (int64_t)Kc * n * W2 * 8 / 16;
+            k_fill_zero<<<blocks_for(y4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(d_Y), y4);
+            AOSX_LAUNCHED();
+            d_X = buf<float2>("b_X", (size_t)G * n * W2);
+            for (int g = 0; g < ngroups; ++g) {
+                Group &gr = groups[g];
+                const int gsz = (int)gr.kidx.size();
+                float *tl = d_tiles + (size_t)gr.slot0 * ff;
+                float *di = d_dinv + (size_t)gr.slot0 * dif;
+                solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_steps, d_X, nullptr);
+                accumulate(gr, false, d_Y, W2, 1);
+                if (p.conjugate_pairs) {
+                    solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_steps, d_X,
+                               nullptr);
+                    accumulate(gr, true, d_Y, W2, 1);
+                }
+            }
+            AOSX_HIP(hipStreamSynchronize(stream));
+            out.stages[9] += now() - tp;
+            const bool more = pass + 1 < p.refinement_passes;
+            Subspace next;
+            compress_and_project(*this, Kc, d_Y, W2, d_H, d_S, p, next, out.stages,
+                                 more ? ritz_names[(pass + 1) & 1] : nullptr);
+            sub = next;
+        }
+        for (int k = 0; k < Kc; ++k) {
+            out.energies[kb + k] = sub.energies[k];
+            out.max_residual[kb + k] = sub.residual[k];
+            out.rank[kb + k] = rank0[k];
+            out.saturated[kb + k] = rank0[k] == L * Ms;
+        }
+    }
+    out.stages[11] += now() - t_start;
+}
+
+}  // namespace aosx

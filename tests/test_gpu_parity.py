@@ -1,0 +1,149 @@
+"""HIP Band path vs the CPU oracle, through the C-ABI (needs an MI355X)."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from aospectrum_amd import synth
+from aospectrum_amd.ciss import CissSettings, HipShiftSolver, band_solve, probe_block
+from aospectrum_amd.data import EnergyInterval
+from aospectrum_amd.sparse import NativeBundle, SparsePair
+from oracle import bloch, ciss as ociss, dense
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*parts):
+    print("[gpu-test]", *parts, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def native_small(small2d):
+    say("create bundle small2d n=832")
+    nb = NativeBundle(small2d, device=0)
+    yield nb
+    nb.close()
+
+
+def test_assembly_matches_oracle(small2d, small2d_pattern, native_small):
+    kpts = np.array([[0.0, 0.0, 0.0], [0.13, 0.37, 0.0], [0.5, 0.5, 0.0]])
+    say("assembly small2d 3 k")
+    h, s = native_small.assemble(kpts)
+    indptr, indices, perm = native_small.pattern()
+    assert np.array_equal(indptr, small2d_pattern.indptr)
+    assert np.array_equal(indices, small2d_pattern.indices)
+    assert np.array_equal(np.sort(perm), np.arange(small2d.n_orbitals))
+    for i, k in enumerate(kpts):
+        ho, so = small2d_pattern.assemble(small2d.operators, k)
+        # complex64 sums of at most a few terms: agreement to float32 rounding
+        assert np.max(np.abs(h[i] - ho)) <= 2e-6 * max(1.0, np.max(np.abs(ho)))
+        assert np.max(np.abs(s[i] - so)) <= 2e-6
+        # exact Hermitian projection: conj pairs and a real diagonal
+        H = small2d_pattern.csr(h[i]).toarray()
+        assert np.array_equal(H, H.conj().T)
+
+
+def test_assembly_duplicate_shifts(tiny3d):
+    """Several lattice shifts on one atom pair are summed (sparse.py:107-113)."""
+    pat = bloch.Pattern.build(tiny3d.operators, tiny3d.basis.atom_offsets, tiny3d.n_orbitals)
+    say("assembly tiny3d")
+    nb = NativeBundle(tiny3d, device=0)
+    try:
+        k = np.array([[0.21, -0.33, 0.4]])
+        h, s = nb.assemble(k)
+        ho, so = pat.assemble(tiny3d.operators, k[0])
+        assert np.max(np.abs(h[0] - ho)) <= 5e-6 * np.max(np.abs(ho))
+        assert np.max(np.abs(s[0] - so)) <= 5e-6
+    finally:
+        nb.close()
+
+
+def _matrices(small2d, small2d_pattern, k):
+    h, s = small2d_pattern.assemble(small2d.operators, k)
+    return SparsePair(small2d_pattern.csr(h), small2d_pattern.csr(s), tuple(k))
+
+
+@pytest.mark.parametrize("precision,steps", [("fp32", 0), ("fp32_ir", 2), ("bf16x3_ir", 3)])
+def test_shift_solve_matches_dense(small2d, small2d_pattern, precision, steps):
+    m = _matrices(small2d, small2d_pattern, [0.13, 0.37, 0.0])
+    n = m.n_orbitals
+    V = probe_block(n, CissSettings())
+    rhs = np.asarray(m.overlap @ V, dtype=np.complex64)
+    z, _, _ = ociss.contour(-2.02, 2.02, ociss.Settings())
+    shifts = z[[0, 7, 15, 16, 31]]
+    say("shift solve", precision, "batch", shifts.size)
+    solver = HipShiftSolver(precision=precision, refinement_steps=steps)
+    try:
+        out = solver.solve_shifted(m, shifts, rhs)
+    finally:
+        solver.close()
+    Hd = m.hamiltonian.toarray().astype(np.complex128)
+    Sd = m.overlap.toarray().astype(np.complex128)
+    for b, zb in enumerate(shifts):
+        A = Hd - complex(zb) * Sd
+        x = np.linalg.solve(A, rhs.astype(np.complex128))
+        err = np.linalg.norm(out.solutions[b] - x) / np.linalg.norm(x)
+        say("  shift", b, "relative error %.2e" % err)
+        # FP32 LU without pivoting: error ~ kappa * eps32; refinement -> ~eps32
+        assert err < (2e-3 if precision == "fp32" else 2e-4)
+
+
+def _oracle_window(small2d, small2d_pattern, k, lo, hi):
+    h, s = small2d_pattern.assemble(small2d.operators, k)
+    H = small2d_pattern.csr(h).toarray()
+    S = small2d_pattern.csr(s).toarray()
+    return dense.window_spectrum(H, S, lo, hi)
+
+
+@pytest.mark.parametrize(
+    "label,settings",
+    [
+        ("plain", CissSettings()),
+        ("plain_pass2", CissSettings(refinement_passes=2)),
+        ("batched_conj", CissSettings(contour_batch_size=16, k_batch=2, conjugate_pairs=True, refinement_passes=2)),
+        (
+            "mixed",
+            CissSettings(
+                contour_batch_size=16,
+                k_batch=2,
+                conjugate_pairs=True,
+                precision="bf16x3_ir",
+                refinement_steps=2,
+                refinement_passes=2,
+            ),
+        ),
+    ],
+)
+def test_band_solve_matches_fp64_oracle(small2d, small2d_pattern, native_small, label, settings):
+    kpts = np.array([[0.13, 0.37, 0.0], [0.0, 0.0, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    say("band solve", label)
+    results, info = band_solve(native_small, settings, interval, kpoints=kpts)
+    for k, r in zip(kpts, results):
+        ref = _oracle_window(small2d, small2d_pattern, k, interval.lower_ev, interval.upper_ev)
+        say(f"  k={k} count={r.energies_ev.size} ref={ref.size} rank={r.subspace_rank} "
+            f"res={r.maximum_residual_ev:.2e} pivots={info['replaced_pivots']}")
+        assert r.energies_ev.size == ref.size
+        assert np.max(np.abs(np.sort(r.energies_ev) - ref)) < 1e-4
+        assert r.complete
+        if settings.refinement_passes >= 2:
+            assert r.maximum_residual_ev <= 1e-4
+
+
+def test_band_solve_matches_oracle_ciss(small2d, small2d_pattern, native_small):
+    """Single-pass plain CISS: same energies and rank as the CPU restatement
+    of the reference algorithm (same probes, same settings)."""
+    k = np.array([0.13, 0.37, 0.0])
+    interval = EnergyInterval(-2.02, 2.02)
+    m = _matrices(small2d, small2d_pattern, k)
+    ref = ociss.solve_interval(m.hamiltonian, m.overlap, interval.lower_ev, interval.upper_ev)
+    say("band solve vs oracle CISS")
+    (r,), _ = band_solve(native_small, CissSettings(), interval, kpoints=k[None, :])
+    assert r.energies_ev.size == ref.energies_ev.size
+    assert np.max(np.abs(r.energies_ev - ref.energies_ev)) < 1e-4
+    assert abs(r.subspace_rank - ref.rank) <= 1
+    # same order of residual as the reference algorithm (both ~1e-3 eV here)
+    assert r.maximum_residual_ev < 5 * ref.maximum_residual_ev + 1e-4
