@@ -70,6 +70,7 @@ class BundleRequest(C.Structure):
         ("s_values", _P),
         ("tile_size", C.c_int32),
         ("ordering", C.c_int32),
+        ("atom_positions", _F64P),
     ]
 
 

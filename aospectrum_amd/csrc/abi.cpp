@@ -86,7 +86,9 @@ int aosx_bundle_create_v1(const aosx_bundle_request_v1 *q, aosx_bundle_result_v1
         return fail(res, AOSX_INT32_OVERFLOW, 0, "bundle dimensions exceed the int32 CSR ABI");
     if (q->value_kind != 0 && q->value_kind != 1) return fail(res, AOSX_INVALID_BLOCKS, 0, "value_kind must be 0 or 1");
     if (q->tile_size != 0 && q->tile_size != 64) return fail(res, AOSX_INVALID_SETTINGS, 0, "tile_size must be 0 or 64");
-    if (q->ordering != 0 && q->ordering != 1) return fail(res, AOSX_INVALID_SETTINGS, 0, "ordering must be 0 or 1");
+    if (q->ordering < 0 || q->ordering > 3) return fail(res, AOSX_INVALID_SETTINGS, 0, "ordering must be 0..3");
+    if (q->ordering == AOSX_ORDERING_NESTED_DISSECTION && !q->atom_positions)
+        return fail(res, AOSX_NULL_POINTER, 0, "nested dissection needs atom positions");
     return guarded(res, [&]() -> int {
         auto b = std::make_unique<Bundle>();
         aosx::BlockTable bt;
@@ -120,7 +122,17 @@ int aosx_bundle_create_v1(const aosx_bundle_request_v1 *q, aosx_bundle_result_v1
         double t1 = now();
         std::vector<int32_t> order;
         int64_t atom_fill = 0;
-        why = aosx::order_atoms(bt, q->ordering, order, atom_fill);
+        const bool use_nd = q->ordering == AOSX_ORDERING_NESTED_DISSECTION ||
+                            (q->ordering == AOSX_ORDERING_AUTO && q->atom_positions);
+        if (use_nd) {
+            for (int64_t i = 0; i < 3 * q->n_atoms; ++i)
+                if (!std::isfinite(q->atom_positions[i]))
+                    return fail(res, AOSX_NONFINITE_VALUES, 0, "atom positions are not finite");
+            why = aosx::order_atoms_nd(bt, q->atom_positions, order);
+            if (why.empty()) atom_fill = aosx::atom_fill(bt, order);
+        } else {
+            why = aosx::order_atoms(bt, q->ordering == AOSX_ORDERING_NATURAL ? 1 : 0, order, atom_fill);
+        }
         if (!why.empty()) return fail(res, AOSX_INVALID_BLOCKS, 0, "%s", why.c_str());
         double t2 = now();
         why = aosx::build_tiles(bt, b->pat, order, 64, b->ts);

@@ -1,8 +1,9 @@
 // Tile triangular solves, RHS packing and FP64 refinement residuals.
 //
-// A right-hand-side block of W columns (W a multiple of 32) is stored per
-// matrix as Nt tiles of TILE rows x W interleaved complex64 values, rows in
-// factor (permuted) order.  Solves run one tile step per launch:
+// A right-hand-side block of W columns (W a multiple of 64) is stored per
+// matrix as Nt row tiles x W/64 planar 64x64 chunks, rows in factor
+// (permuted) order.  Solves run one tile step per launch, with the FP32 MFMA
+// tile core of the factorisation:
 //   diag   : Y_k <- D_k Y_k with D_k = inv(L_kk), inv(U_kk) or their
 //            conjugate transposes (for A^H = U^H L^H, used by conjugate shifts);
 //   update : Y_i -= T_ik Y_k for the tiles i coupled to k.
@@ -11,111 +12,104 @@
 
 namespace aosx {
 
-// Load a stored tile (planar) into LDS as row-major re/im with stride LDA,
-// optionally as its conjugate transpose.
-__device__ __forceinline__ void stage_tile(const float *__restrict__ g, float *Tre, float *Tim, bool ctrans,
-                                           int tid) {
+// RHS layout of the solves: per matrix, Nt row tiles x (W / 64) column chunks,
+// each chunk a 64 x 64 complex block stored like a factor tile (re plane, im
+// plane, row-major), so tile products reuse the MFMA core of the factorisation.
+__device__ __forceinline__ int64_t rhs_block(int64_t mat_stride, int m, int tile, int nchunk, int c) {
+    return (int64_t)m * mat_stride + ((int64_t)tile * nchunk + c) * TILE_FLOATS;
+}
+
+// Stage the conjugate transpose of a stored tile as the A operand.
+__device__ __forceinline__ void stage_a_ctrans(const float *__restrict__ g, float *Are, float *Aim, int tid) {
     for (int e = tid; e < TILE2; e += 256) {
         const int r = e / TILE, c = e % TILE;
-        const float re = g[e], im = g[PLANE + e];
-        if (ctrans) { Tre[c * LDA + r] = re; Tim[c * LDA + r] = -im; }
-        else { Tre[r * LDA + c] = re; Tim[r * LDA + c] = im; }
+        Are[c * LDA + r] = g[e];
+        Aim[c * LDA + r] = -g[PLANE + e];
     }
 }
 
-// out[r][c] = sum_j T[r][j] Y[j][c] for this thread's 8 rows (r0 + 8q) and column c.
-__device__ __forceinline__ void apply_tile(const float *Tre, const float *Tim, const float2 *Ys, int r0, int c,
-                                           float2 *out) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) out[q] = make_float2(0.f, 0.f);
-    for (int j = 0; j < TILE; ++j) {
-        const float2 y = Ys[j * 32 + c];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int r = r0 + 8 * q;
-            const float tr = Tre[r * LDA + j], ti = Tim[r * LDA + j];
-            out[q].x += tr * y.x - ti * y.y;
-            out[q].y += tr * y.y + ti * y.x;
-        }
-    }
-}
-
-// which: 0 inv(L_kk), 1 inv(U_kk); ctrans applies the conjugate transpose.
-__global__ void __launch_bounds__(256) k_solve_diag(float2 *__restrict__ y, int64_t y_mat_stride, int W,
+// Y_k <- D_k Y_k for all chunks; D_k = inv(L_kk) (which 0) or inv(U_kk)
+// (which 1), conjugate-transposed when ctrans.
+__global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                    const float *__restrict__ dinv, int nt, int k, int which,
                                                    int ctrans) {
-    __shared__ float Tre[TILE * LDA], Tim[TILE * LDA];
-    __shared__ float2 Ys[TILE * 32];
-    const int m = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+    const int m = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
-    stage_tile(d, Tre, Tim, ctrans != 0, tid);
-    float2 *yk = y + (int64_t)m * y_mat_stride + (int64_t)k * TILE * W + chunk * 32;
-    for (int e = tid; e < TILE * 32; e += 256) Ys[e] = yk[(e / 32) * W + e % 32];
-    __syncthreads();
-    const int c = tid & 31, r0 = tid >> 5;
-    float2 out[8];
-    apply_tile(Tre, Tim, Ys, r0, c, out);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) yk[(r0 + 8 * q) * W + c] = out[q];
+    if (ctrans) stage_a_ctrans(d, Are, Aim, tid);
+    else stage_a(d, Are, Aim, tid);
+    for (int c = 0; c < nchunk; ++c) {
+        float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
+        __syncthreads();
+        stage_b(yk, Bre, Bim, tid);
+        __syncthreads();
+        floatx16 cre = {}, cim = {};
+        mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        store_acc(yk, cre, cim, wave, lane);
+    }
 }
 
-// Y_i -= T Y_k for i = list[blockIdx.y]; T is tile (i,k) or, with ctrans, the
-// conjugate transpose of tile (k,i).
-__global__ void __launch_bounds__(256) k_solve_update(float2 *__restrict__ y, int64_t y_mat_stride, int W,
+// Y_i -= T Y_k for i = list[blockIdx.x], all chunks; T is tile (i,k) or,
+// with ctrans, the conjugate transpose of tile (k,i).
+__global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                      const float *__restrict__ tiles, int64_t mat_stride,
                                                      const int32_t *__restrict__ tile_id, int nt, int k,
                                                      const int32_t *__restrict__ list, int ctrans) {
-    __shared__ float Tre[TILE * LDA], Tim[TILE * LDA];
-    __shared__ float2 Ys[TILE * 32];
-    const int m = blockIdx.z, chunk = blockIdx.x, tid = threadIdx.x;
-    const int i = list[blockIdx.y];
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int i = list[blockIdx.x];
     const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
-    stage_tile(tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS, Tre, Tim, ctrans != 0, tid);
-    const float2 *yk = y + (int64_t)m * y_mat_stride + (int64_t)k * TILE * W + chunk * 32;
-    for (int e = tid; e < TILE * 32; e += 256) Ys[e] = yk[(e / 32) * W + e % 32];
-    __syncthreads();
-    const int c = tid & 31, r0 = tid >> 5;
-    float2 out[8];
-    apply_tile(Tre, Tim, Ys, r0, c, out);
-    float2 *yi = y + (int64_t)m * y_mat_stride + (int64_t)i * TILE * W + chunk * 32;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        float2 v = yi[(r0 + 8 * q) * W + c];
-        v.x -= out[q].x;
-        v.y -= out[q].y;
-        yi[(r0 + 8 * q) * W + c] = v;
+    const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+    if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
+    else stage_a(t, Are, Aim, tid);
+    for (int c = 0; c < nchunk; ++c) {
+        const float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
+        float *yi = y + rhs_block(y_mat_stride, m, i, nchunk, c);
+        __syncthreads();
+        stage_b(yk, Bre, Bim, tid);
+        floatx16 cre, cim;
+        load_acc(yi, cre, cim, wave, lane);
+        __syncthreads();
+        mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        store_acc(yi, cre, cim, wave, lane);
     }
 }
 
-// Y[m] (tiled, permuted, W columns) <- src[src_index[m]] (row-major n x w_src);
-// columns >= w_src and padded rows become zero.
-__global__ void k_pack_rhs(float2 *__restrict__ y, int64_t y_mat_stride, int W, const float2 *__restrict__ src,
+// Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
+// (row-major n x w_src complex64); columns >= w_src and padded rows are zero.
+__global__ void k_pack_rhs(float *__restrict__ y, int64_t y_mat_stride, int nchunk, const float2 *__restrict__ src,
                            int w_src, const int32_t *__restrict__ src_index, const int32_t *__restrict__ perm,
                            int64_t n, int64_t padded) {
     const int m = blockIdx.y;
     const float2 *s = src + (int64_t)src_index[m] * n * w_src;
-    float2 *ym = y + (int64_t)m * y_mat_stride;
+    const int W = nchunk * TILE;
     const int64_t total = padded * W, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const int64_t row = e / W;
-        const int c = (int)(e % W);
+        const int col = (int)(e % W);
         float2 v = make_float2(0.f, 0.f);
-        if (row < n && c < w_src) v = s[(int64_t)perm[row] * w_src + c];
-        ym[e] = v;
+        if (row < n && col < w_src) v = s[(int64_t)perm[row] * w_src + col];
+        float *blk = y + rhs_block(y_mat_stride, m, (int)(row / TILE), nchunk, col / TILE);
+        const int off = (int)(row % TILE) * TILE + col % TILE;
+        blk[off] = v.x;
+        blk[PLANE + off] = v.y;
     }
 }
 
 // dst[m] (row-major n x w_dst, original order) <- Y[m]
-__global__ void k_unpack(float2 *__restrict__ dst, int w_dst, const float2 *__restrict__ y, int64_t y_mat_stride,
-                         int W, const int32_t *__restrict__ perm, int64_t n) {
+__global__ void k_unpack(float2 *__restrict__ dst, int w_dst, const float *__restrict__ y, int64_t y_mat_stride,
+                         int nchunk, const int32_t *__restrict__ perm, int64_t n) {
     const int m = blockIdx.y;
-    const float2 *ym = y + (int64_t)m * y_mat_stride;
     float2 *d = dst + (int64_t)m * n * w_dst;
     const int64_t total = n * w_dst, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const int64_t row = e / w_dst;
-        const int c = (int)(e % w_dst);
-        d[(int64_t)perm[row] * w_dst + c] = ym[row * W + c];
+        const int col = (int)(e % w_dst);
+        const float *blk = y + rhs_block(y_mat_stride, m, (int)(row / TILE), nchunk, col / TILE);
+        const int off = (int)(row % TILE) * TILE + col % TILE;
+        d[(int64_t)perm[row] * w_dst + col] = make_float2(blk[off], blk[PLANE + off]);
     }
 }
 

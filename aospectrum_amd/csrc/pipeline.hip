@@ -239,40 +239,40 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
 }
 
 // ------------------------------------------------------------------- solves
-void Bundle::solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans) {
-    const int nt = ts.n_diag, chunks = W / 32;
-    const int64_t ystride = (int64_t)nt * 64 * W;
+// y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
+void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans) {
+    float *y = reinterpret_cast<float *>(y2);
+    const int nt = ts.n_diag, nchunk = (W + 63) / 64;
+    const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
     EventTimer tm(kstats.enabled, stream);
     for (int k = 0; k < nt; ++k) {
-        k_solve_diag<<<dim3(chunks, nmat), 256, 0, stream>>>(y, ystride, W, dinv, nt, k, ctrans ? 1 : 0,
-                                                             ctrans ? 1 : 0);
+        k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
         AOSX_LAUNCHED();
         const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
         if (s) {
-            k_solve_update<<<dim3(chunks, s, nmat), 256, 0, stream>>>(y, ystride, W, tiles, mstride, d_tile_id,
-                                                                      nt, k, d_next + ts.next_ptr[k],
-                                                                      ctrans ? 1 : 0);
+            k_solve_update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
             AOSX_LAUNCHED();
         }
     }
     for (int k = nt - 1; k >= 0; --k) {
-        k_solve_diag<<<dim3(chunks, nmat), 256, 0, stream>>>(y, ystride, W, dinv, nt, k, ctrans ? 0 : 1,
-                                                             ctrans ? 1 : 0);
+        k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
         AOSX_LAUNCHED();
         const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
         if (s) {
-            k_solve_update<<<dim3(chunks, s, nmat), 256, 0, stream>>>(y, ystride, W, tiles, mstride, d_tile_id,
-                                                                      nt, k, d_prev + ts.prev_ptr[k],
-                                                                      ctrans ? 1 : 0);
+            k_solve_update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
             AOSX_LAUNCHED();
         }
     }
     if (kstats.enabled) {
         kstats.solve_ms += tm.stop();
         kstats.solve_launches += 1;
-        // every stored tile is read once per solve, every RHS tile twice
-        kstats.solve_bytes += (double)nmat * ((double)ts.n_tiles * 2 * 4 * 4096 + 4.0 * nt * 64 * W * 8);
+        // algorithmic bytes: every stored tile is read once per sweep pair,
+        // every RHS block is read and written once per tile it meets
+        kstats.solve_bytes += (double)nmat * ((double)ts.n_tiles * TILE_FLOATS * 4.0 +
+                                              2.0 * (double)ts.next_idx.size() * nchunk * TILE_FLOATS * 4.0 * 3.0);
     }
 }
 
@@ -280,10 +280,10 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
                         const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                         const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
                         float2 *x_out, double *rel_residual) {
-    const int nt = ts.n_diag;
+    const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t padded = (int64_t)nt * 64;
-    const int64_t ystride = padded * W;
-    float2 *y = buf<float2>("s_y", (size_t)nmat * ystride);
+    const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
+    float *y = buf<float>("s_y", (size_t)nmat * ystride);
     int32_t *d_ri = buf<int32_t>("s_rhs_index", nmat);
     int32_t *d_id = buf<int32_t>("s_identity", nmat);
     int32_t *d_ki = buf<int32_t>("s_kidx", nmat);
@@ -294,11 +294,13 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     AOSX_HIP(hipMemcpyAsync(d_id, ident.data(), nmat * 4, hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_ki, kidx.data(), nmat * 4, hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_z, zeff.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
-    const int pb = blocks_for(padded * W, 256, 4096);
-    k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, W, d_rhs, W, d_ri, d_perm, n, padded);
+    const int pb = blocks_for(padded * nchunk * 64, 256, 4096);
+    const int ub = blocks_for(n * W, 256, 4096);
+    float2 *y2 = reinterpret_cast<float2 *>(y);
+    k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, d_rhs, W, d_ri, d_perm, n, padded);
     AOSX_LAUNCHED();
-    solve(nmat, W, y, tiles, dinv, ctrans);
-    k_unpack<<<dim3(blocks_for(n * W, 256, 4096), nmat), 256, 0, stream>>>(x_out, W, y, ystride, W, d_perm, n);
+    solve(nmat, W, y2, tiles, dinv, ctrans);
+    k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(x_out, W, y, ystride, nchunk, d_perm, n);
     AOSX_LAUNCHED();
     if (ir_steps <= 0 && !rel_residual) return;
     float2 *r = buf<float2>("s_r", (size_t)nmat * n * W);
@@ -307,7 +309,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     for (int step = 0; step <= ir_steps; ++step) {
         const bool last = step == ir_steps;
         if (last && !rel_residual) break;
-        k_fill_zero<<<1, 64, 0, stream>>>(reinterpret_cast<float4 *>(d_norm), (int64_t)nmat * 2 * 8 / 16);
+        k_fill_zero<<<1, 64, 0, stream>>>(reinterpret_cast<float4 *>(d_norm), (int64_t)nmat);
         AOSX_LAUNCHED();
         k_residual_shifted<<<dim3(rb, nmat), 256, 0, stream>>>(r, d_rhs, d_ri, x_out, W, d_h, d_s, d_ki, d_z,
                                                                d_indptr, d_indices, n, nnz, d_norm, d_norm + nmat);
@@ -322,10 +324,10 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
             *rel_residual = worst;
             break;
         }
-        k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, W, r, W, d_id, d_perm, n, padded);
+        k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, r, W, d_id, d_perm, n, padded);
         AOSX_LAUNCHED();
-        solve(nmat, W, y, tiles, dinv, ctrans);
-        k_unpack<<<dim3(blocks_for(n * W, 256, 4096), nmat), 256, 0, stream>>>(r, W, y, ystride, W, d_perm, n);
+        solve(nmat, W, y2, tiles, dinv, ctrans);
+        k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(r, W, y, ystride, nchunk, d_perm, n);
         AOSX_LAUNCHED();
         k_axpy<<<blocks_for((int64_t)nmat * n * W, 256, 8192), 256, 0, stream>>>(x_out, r, (int64_t)nmat * n * W);
         AOSX_LAUNCHED();

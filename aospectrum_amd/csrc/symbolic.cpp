@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <numeric>
 
@@ -238,6 +239,127 @@ std::string order_atoms(const BlockTable &bt, int ordering, std::vector<int32_t>
         std::fill(adj.row(v), adj.row(v) + adj.words, 0ull);
     }
     return "";
+}
+
+// Nested dissection on the atom graph.  A region is split at the median
+// coordinate of its widest axis; the separator is a vertex cover of the graph
+// edges that cross the split (greedy, highest remaining cut degree first), so
+// periodic-image edges -- which the block table carries -- are separated too.
+// Regions are ordered before their separator; regions of at most `leaf` atoms
+// keep coordinate order.
+std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<int32_t> &order) {
+    const int64_t na = bt.n_atoms;
+    std::vector<std::vector<int32_t>> adj(na);
+    for (int64_t b = 0; b < bt.n_blocks; ++b) {
+        const int32_t i = bt.row_atom[b], j = bt.col_atom[b];
+        if (i != j) adj[i].push_back(j);
+    }
+    for (auto &a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<int32_t> side(na, -1);  // region-local marks: 0 A, 1 B, -1 outside
+    std::vector<int32_t> cutdeg(na, 0);
+    std::vector<char> covered(na, 0);
+    const size_t leaf = 12;
+    order.clear();
+    order.reserve(na);
+    struct Frame {
+        std::vector<int32_t> set;
+    };
+    // explicit recursion: emit(region) = emit(A'), emit(B'), separator
+    std::function<void(std::vector<int32_t> &)> nd = [&](std::vector<int32_t> &set) {
+        if (set.size() <= leaf) {
+            order.insert(order.end(), set.begin(), set.end());
+            return;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int32_t a : set)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], pos[3 * a + d]);
+                hi[d] = std::max(hi[d], pos[3 * a + d]);
+            }
+        int axis = 0;
+        for (int d = 1; d < 3; ++d)
+            if (hi[d] - lo[d] > hi[axis] - lo[axis]) axis = d;
+        std::stable_sort(set.begin(), set.end(),
+                         [&](int32_t x, int32_t y) { return pos[3 * x + axis] < pos[3 * y + axis]; });
+        const size_t half = set.size() / 2;
+        for (size_t q = 0; q < set.size(); ++q) side[set[q]] = q < half ? 0 : 1;
+        // cut graph and greedy vertex cover
+        std::vector<int32_t> cutv;
+        for (int32_t a : set) {
+            int c = 0;
+            for (int32_t nb : adj[a])
+                if (side[nb] >= 0 && side[nb] != side[a]) ++c;
+            cutdeg[a] = c;
+            if (c) cutv.push_back(a);
+        }
+        std::vector<int32_t> sep;
+        while (true) {
+            int32_t best = -1;
+            for (int32_t v : cutv)
+                if (!covered[v] && cutdeg[v] > 0 && (best < 0 || cutdeg[v] > cutdeg[best])) best = v;
+            if (best < 0) break;
+            covered[best] = 1;
+            sep.push_back(best);
+            for (int32_t nb : adj[best])
+                if (side[nb] >= 0 && side[nb] != side[best] && !covered[nb]) cutdeg[nb]--;
+            cutdeg[best] = 0;
+        }
+        std::vector<int32_t> A, B;
+        for (int32_t a : set)
+            if (!covered[a]) (side[a] == 0 ? A : B).push_back(a);
+        for (int32_t a : set) {
+            side[a] = -1;
+            cutdeg[a] = 0;
+        }
+        for (int32_t a : sep) covered[a] = 0;
+        if (A.empty() || B.empty() || sep.size() * 10 > set.size() * 9) {
+            order.insert(order.end(), set.begin(), set.end());
+            return;
+        }
+        std::stable_sort(sep.begin(), sep.end(),
+                         [&](int32_t x, int32_t y) { return pos[3 * x + axis] < pos[3 * y + axis]; });
+        nd(A);
+        nd(B);
+        order.insert(order.end(), sep.begin(), sep.end());
+    };
+    std::vector<int32_t> all(na);
+    std::iota(all.begin(), all.end(), 0);
+    nd(all);
+    if ((int64_t)order.size() != na) return "nested dissection lost atoms";
+    return "";
+}
+
+// nnz(L+U) of the atom-level symbolic factor for a given elimination order.
+int64_t atom_fill(const BlockTable &bt, const std::vector<int32_t> &order) {
+    const int64_t na = bt.n_atoms;
+    std::vector<int32_t> rank(na);
+    for (int64_t q = 0; q < na; ++q) rank[order[q]] = (int32_t)q;
+    Bitset g;
+    g.resize(na, na);
+    for (int64_t b = 0; b < bt.n_blocks; ++b) {
+        const int32_t i = rank[bt.row_atom[b]], j = rank[bt.col_atom[b]];
+        g.set(i, j);
+        g.set(j, i);
+    }
+    int64_t fill = 0;
+    std::vector<int64_t> nbr;
+    for (int64_t k = 0; k < na; ++k) {
+        const int64_t wk = bt.atom_offsets[order[k] + 1] - bt.atom_offsets[order[k]];
+        nbr.clear();
+        for_each_bit(g.row(k), g.words, k + 1, [&](int64_t i) { nbr.push_back(i); });
+        int64_t deg = 0;
+        for (int64_t i : nbr) deg += bt.atom_offsets[order[i] + 1] - bt.atom_offsets[order[i]];
+        fill += wk * wk + 2 * wk * deg;
+        const uint64_t *rk = g.row(k);
+        for (int64_t i : nbr) {
+            uint64_t *ri = g.row(i);
+            for (int64_t q = (k + 1) >> 6; q < g.words; ++q) ri[q] |= rk[q];
+        }
+    }
+    return fill;
 }
 
 std::string build_tiles(const BlockTable &bt, const Pattern &pat, const std::vector<int32_t> &atom_order,

@@ -66,6 +66,8 @@ struct TileStructure {
 std::string build_pattern(const BlockTable &blocks, Pattern &out, int &code);
 std::string order_atoms(const BlockTable &blocks, int ordering, std::vector<int32_t> &order,
                         int64_t &atom_fill_entries);
+std::string order_atoms_nd(const BlockTable &blocks, const double *positions, std::vector<int32_t> &order);
+int64_t atom_fill(const BlockTable &blocks, const std::vector<int32_t> &order);
 std::string build_tiles(const BlockTable &blocks, const Pattern &pattern,
                         const std::vector<int32_t> &atom_order, int tile, TileStructure &out);
 

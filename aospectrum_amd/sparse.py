@@ -45,7 +45,7 @@ class NativeBundle:
         bundle: LocalizedOperatorBundle,
         *,
         device: int = 0,
-        ordering: str = "minimum_degree",
+        ordering: str = "auto",
     ) -> None:
         ops = bundle.operators
         self.n = bundle.n_orbitals
@@ -59,6 +59,7 @@ class NativeBundle:
             block_offsets=np.ascontiguousarray(ops.block_offsets, dtype=np.int64),
             h=np.ascontiguousarray(ops.h_values, dtype=kind),
             s=np.ascontiguousarray(ops.s_values, dtype=kind),
+            positions=np.ascontiguousarray(bundle.structure.positions_angstrom, dtype=np.float64),
         )
         self._create(bundle.basis.n_atoms, ops.n_blocks, 0 if real else 1, device, ordering)
 
@@ -81,6 +82,7 @@ class NativeBundle:
             block_offsets=np.arange(nnz + 1, dtype=np.int64),
             h=np.ones(nnz, dtype=np.float32),
             s=np.ones(nnz, dtype=np.float32),
+            positions=None,
         )
         self._create(n, nnz, 0, device, "minimum_degree")
         return self
@@ -102,7 +104,9 @@ class NativeBundle:
         req.h_values = a["h"].ctypes.data
         req.s_values = a["s"].ctypes.data
         req.tile_size = 0
-        req.ordering = {"minimum_degree": 0, "natural": 1}[ordering]
+        req.ordering = {"auto": 0, "natural": 1, "minimum_degree": 2, "nested_dissection": 3}[ordering]
+        if a["positions"] is not None:
+            req.atom_positions = N.ptr(a["positions"], C.c_double)
         res = N.BundleResult()
         handle = C.c_void_p()
         rc = lib.aosx_bundle_create_v1(C.byref(req), C.byref(res), C.byref(handle))

@@ -92,7 +92,7 @@ SPECS: dict[str, SynthSpec] = {
         cluster_count=96,
     ),
     "bulk3d": SynthSpec(
-        "bulk3d", (10, 10, 10), 2.4, (24.0, 24.0, 24.0), (True, True, True), 1002, gap=2.7
+        "bulk3d", (10, 10, 10), 2.4, (24.0, 24.0, 24.0), (True, True, True), 1002, gap=2.85
     ),
     "slab52k": SynthSpec(
         "slab52k", (20, 20, 10), 2.4, (48.0, 48.0, 44.0), (True, True, False), 1003, gap=7.0

@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""Band-path benchmark on one MI355X.
+
+Workload: the 13,000-orbital 3D bulk bundle (1,000 atoms on a jittered
+10x10x10 grid, 13 AOs per atom, 6 Angstrom cutoff over R in {-1,0,1}^3),
+the 121-point Y-G-X-M path and the [-1, 1] eV window (solve interval
+[-1.02, 1.02] eV), CISS with 32 contour points, block 32, moment 4.
+
+One "step" is one batch of ``--k-batch`` k points through the whole path
+(on-device assembly, all shifted factorisations and solves, moments,
+compression, Rayleigh-Ritz, filter passes and FP64 residuals) in the batched
+mixed-precision configuration.  The same k points are also solved with the
+plain configuration (one shift per factorisation launch, one k point at a
+time, FP32 LU) for the speedup figure.  Both use two filter passes so that
+both meet the 1e-4 eV residual bar; ``value`` is the batched mixed k-points/s.
+
+The roofline entry reports the dominant kernel (the Schur-update GEMM),
+timed with HIP events on the library's stream over the timed steps; its
+algorithmic flops are 8 real flops per complex multiply-add of the tile
+updates.  The CPU baseline runs the oracle's CISS shifted solves (dense
+complex64 LAPACK getrf/getrs, the cheaper CPU method at this fill) for a
+bounded sample of the same workload on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0}
+
+
+def log(*parts):
+    print("[bench]", *parts, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(bundle, kpoint, interval, shifts_sampled: int) -> dict:
+    """Time `shifts_sampled` of the 32 shifted solves of one k point with the
+    oracle's dense LAPACK solver and scale to k-points/s."""
+    from oracle import bloch, ciss as ociss
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pat = bloch.Pattern.build(bundle.operators, bundle.basis.atom_offsets, bundle.n_orbitals)
+    h, s = pat.assemble(bundle.operators, kpoint)
+    H, S = pat.csr(h), pat.csr(s)
+    st = ociss.Settings()
+    z, _, _ = ociss.contour(interval.lower_ev, interval.upper_ev, st)
+    V = ociss.probes(bundle.n_orbitals, st)
+    rhs = np.asarray(S @ V, dtype=np.complex64)
+    t0 = time.perf_counter()
+    for j in range(shifts_sampled):
+        ociss.lapack_solver(H, S, z[j], rhs)
+    dt = time.perf_counter() - t0
+    per_k = dt / shifts_sampled * st.integration_points
+    return {
+        "value": 1.0 / per_k,
+        "unit": "k-points/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{shifts_sampled} of the 32 shifted factorisations+solves of one k point "
+        f"(dense complex64 LAPACK, {cores} threads, {dt:.1f} s), scaled x{st.integration_points / shifts_sampled:.0f}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="bulk3d")
+    ap.add_argument("--k-batch", type=int, default=4)
+    ap.add_argument("--plain-kpoints", type=int, default=2)
+    ap.add_argument("--precision", default="bf16x3_ir")
+    ap.add_argument("--ir-steps", type=int, default=2)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--cpu-shifts", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: the Band path batches k points on a single MI355X "
+            "and is not distributed across GPUs (run with --gpus 1)",
+            file=sys.stderr,
+        )
+        return 2
+
+    from aospectrum_amd import synth
+    from aospectrum_amd.band import ygxm_path
+    from aospectrum_amd.ciss import CissSettings, band_solve
+    from aospectrum_amd.data import EnergyInterval
+    from aospectrum_amd.sparse import NativeBundle
+
+    windows = {"bulk3d": (-1.0, 1.0), "small2d": (-2.0, 2.0), "stress2d": (-1.5, 1.5)}
+    spec = synth.SPECS[args.workload]
+    lo, hi = windows.get(args.workload, (-1.0, 1.0))
+    interval = EnergyInterval(lo - 0.02, hi + 0.02)
+    t0 = time.perf_counter()
+    bundle = synth.generate(spec)
+    gen_s = time.perf_counter() - t0
+    log(f"generated {spec.name}: n={bundle.n_orbitals} blocks={bundle.operators.n_blocks} ({gen_s:.1f} s)")
+    t0 = time.perf_counter()
+    native = NativeBundle(bundle, device=0)
+    setup_s = time.perf_counter() - t0
+    info = native.info
+    log(
+        f"symbolic+upload {setup_s:.1f} s: nnz={info['nnz']} tiles={info['n_factor_tiles']} "
+        f"flops/factor={info['factor_flops']:.3e}"
+    )
+    path = ygxm_path(40)
+    K = args.k_batch
+    mixed = CissSettings(
+        contour_batch_size=16,
+        k_batch=K,
+        conjugate_pairs=True,
+        precision=args.precision,
+        refinement_steps=args.ir_steps,
+        refinement_passes=args.passes,
+    )
+    # plain: the reference's configuration -- one shift per factorisation
+    # launch, one k point at a time, FP32 LU with two refinement steps (the
+    # reference's cuDSS call sets IR_N_STEPS = 2)
+    plain = CissSettings(precision="fp32_ir", refinement_steps=2, refinement_passes=args.passes)
+
+    def kslice(step):
+        # timed steps walk the path from its start; warmup steps (negative)
+        # come from its end
+        start = (step * K) % path.n_points
+        idx = [(start + i) % path.n_points for i in range(K)]
+        return idx, path.fractional_points[idx]
+
+    for w in range(args.warmup):
+        _, kp = kslice(-(w + 1))
+        band_solve(native, mixed, interval, kpoints=kp)
+        log(f"warmup step {w} done")
+    native.kernel_stats(True)
+    lat = []
+    results = {}
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        idx, kp = kslice(s)
+        t = time.perf_counter()
+        res, binfo = band_solve(native, mixed, interval, kpoints=kp)
+        lat.append(time.perf_counter() - t)
+        for i, r in zip(idx, res):
+            results[i] = r
+        log(f"step {s}: {lat[-1]:.2f} s, counts {[r.energies_ev.size for r in res]}, "
+            f"ranks {[r.subspace_rank for r in res]}, max res {max(r.maximum_residual_ev for r in res):.2e}, "
+            f"stages {{{', '.join(f'{k}: {v:.2f}' for k, v in binfo['stage_seconds'].items() if v > 0.01)}}}")
+    total = time.perf_counter() - t_start
+    ks = native.kernel_stats(False)
+    nk_done = args.steps * K
+    value = nk_done / total
+
+    plain_rate = None
+    plain_info = {}
+    if not args.no_plain and args.plain_kpoints > 0:
+        pidx = sorted(results)[: args.plain_kpoints]
+        native.kernel_stats(True)
+        t = time.perf_counter()
+        pres, _ = band_solve(native, plain, interval, kpoints=path.fractional_points[pidx])
+        pdt = time.perf_counter() - t
+        pks = native.kernel_stats(False)
+        plain_rate = len(pidx) / pdt
+        mismatch = [i for i, r in zip(pidx, pres) if r.energies_ev.size != results[i].energies_ev.size]
+        dmax = max(
+            (float(np.max(np.abs(r.energies_ev - results[i].energies_ev))) for i, r in zip(pidx, pres)
+             if r.energies_ev.size == results[i].energies_ev.size and r.energies_ev.size),
+            default=0.0,
+        )
+        plain_info = {
+            "plain_kpoints_per_s": plain_rate,
+            "plain_seconds_per_kpoint": pdt / len(pidx),
+            "plain_max_residual_ev": max(r.maximum_residual_ev for r in pres),
+            "plain_vs_mixed_count_mismatch": mismatch,
+            "plain_vs_mixed_max_dE_ev": dmax,
+            "plain_schur_tflops": pks["schur_flops"] / max(pks["schur_ms"], 1e-9) / 1e9,
+        }
+        log(f"plain: {pdt / len(pidx):.2f} s per k point; mixed/plain = {value / plain_rate:.2f}x")
+
+    golden = {}
+    gpath = ROOT / "tests" / "golden" / f"{args.workload}_counts.json"
+    if gpath.is_file():
+        g = json.loads(gpath.read_text())
+        for kp, c in zip(g["kpoints"], g["counts"]):
+            for i, r in results.items():
+                if np.allclose(path.fractional_points[i], kp):
+                    golden[i] = (c, int(r.energies_ev.size))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(bundle, path.fractional_points[0], interval, args.cpu_shifts)
+        log(f"cpu baseline {cpu['value']:.2e} k-points/s")
+
+    achieved = ks["schur_flops"] / max(ks["schur_ms"], 1e-9) / 1e9  # TFLOP/s
+    peak = PEAK_TFLOPS[args.precision]
+    lat_ms = np.asarray(lat) * 1e3
+    line = {
+        "metric": "k-points/s at residual <=1e-4 eV, batched-mixed vs plain CISS; p50/p99 latency",
+        "value": value,
+        "unit": "k-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(total / args.steps * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64" if args.precision.startswith("fp32") else f"complex64+{args.precision.split('_')[0]}",
+        "data": "synthetic (seeded generator, no datasets)",
+        "config": {
+            "workload": f"{spec.name}: n={bundle.n_orbitals}, {bundle.structure.n_atoms} atoms, "
+            f"nnz={info['nnz']}, 121-pt YGXM path, window [{lo},{hi}] eV",
+            "integration_points": 32,
+            "block_size": 32,
+            "moment_size": 4,
+            "k_batch": K,
+            "precision": args.precision,
+            "ir_steps": args.ir_steps,
+            "filter_passes": args.passes,
+            "conjugate_pairs": True,
+            "parallelism": "single GPU, k points batched on device",
+        },
+        "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
+        **plain_info,
+        "p50_step_ms": float(np.percentile(lat_ms, 50)),
+        "p99_step_ms": float(np.percentile(lat_ms, 99)),
+        "max_residual_ev": max(r.maximum_residual_ev for r in results.values()),
+        "in_window_counts": {str(i): int(r.energies_ev.size) for i, r in sorted(results.items())},
+        "subspace_ranks": {str(i): int(r.subspace_rank) for i, r in sorted(results.items())},
+        "saturated": [i for i, r in sorted(results.items()) if not r.complete],
+        "golden_counts": {str(i): {"oracle": c, "gpu": g} for i, (c, g) in golden.items()},
+        "factor_tiles": info["n_factor_tiles"],
+        "factor_flops_per_matrix": info["factor_flops"],
+        "roofline": {
+            "bound": "mfma",
+            "kernel": f"k_schur ({args.precision})",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "launches": ks["schur_launches"],
+            "avg_launch_ms": ks["schur_ms"] / max(ks["schur_launches"], 1),
+        },
+        "kernel_ms": {k: v for k, v in ks.items() if k.endswith("_ms")},
+        "cpu_baseline": cpu,
+        "setup_seconds": {"generate": gen_s, "symbolic_and_upload": setup_s},
+    }
+    native.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -80,6 +80,14 @@ enum {
     AOSX_PRECISION_BF16_IR = 3     /* Schur updates in one BF16 MFMA product + IR      */
 };
 
+/* Fill-reducing ordering of the atom graph. */
+enum {
+    AOSX_ORDERING_AUTO = 0,             /* nested dissection with positions, else minimum degree */
+    AOSX_ORDERING_NATURAL = 1,
+    AOSX_ORDERING_MINIMUM_DEGREE = 2,
+    AOSX_ORDERING_NESTED_DISSECTION = 3 /* needs atom_positions                            */
+};
+
 typedef struct {
     float real;
     float imag;
@@ -104,7 +112,9 @@ typedef struct {
     const void *h_values;          /* [block_offsets[n_blocks]]                    */
     const void *s_values;
     int32_t tile_size;             /* factor tile edge: 0 (default 64) or 64       */
-    int32_t ordering;              /* 0: minimum degree on the atom graph; 1: natural */
+    int32_t ordering;              /* AOSX_ORDERING_*                              */
+    const double *atom_positions;  /* [n_atoms][3] Cartesian, or NULL; steers the
+                                      nested-dissection split planes            */
 } aosx_bundle_request_v1;
 
 typedef struct {

@@ -88,6 +88,16 @@ def superlu_solver(H: sp.csr_matrix, S: sp.csr_matrix, z: complex, rhs: np.ndarr
     return np.asarray(lu.solve(rhs), dtype=np.complex64)
 
 
+def lapack_solver(H, S, z, rhs):
+    """Dense complex64 LAPACK getrf/getrs of (H - z S) -- the CPU-baseline
+    shifted solver for near-dense fill (cheaper than SuperLU there)."""
+    a = H.toarray() if hasattr(H, "toarray") else np.asarray(H)
+    b = S.toarray() if hasattr(S, "toarray") else np.asarray(S)
+    a = np.asarray(a, dtype=np.complex64) - np.complex64(z) * np.asarray(b, dtype=np.complex64)
+    lu = sl.lu_factor(a, overwrite_a=True, check_finite=False)
+    return np.asarray(sl.lu_solve(lu, rhs.astype(np.complex64), check_finite=False), dtype=np.complex64)
+
+
 def dense_solver(H, S, z, rhs):
     a = (H.toarray().astype(np.complex128) - complex(z) * S.toarray().astype(np.complex128))
     return np.linalg.solve(a, rhs.astype(np.complex128)).astype(np.complex64)
