@@ -176,6 +176,9 @@ class BandRequest(C.Structure):
         ("rank_relative_tolerance", C.c_double),
         ("spurious_relative_threshold", C.c_double),
         ("probes", _C64P),
+        ("augmentation", _C64P),
+        ("augmentation_width", C.c_int32),
+        ("reserved_aug", C.c_int32),
         ("precision", C.c_int32),
         ("refinement_steps", C.c_int32),
         ("refinement_passes", C.c_int32),

@@ -112,14 +112,18 @@ def contour_geometry(interval: EnergyInterval, settings: CissSettings):
 _PROBE_CACHE: dict[tuple[int, int, int], np.ndarray] = {}
 
 
-def probe_block(n: int, settings: CissSettings) -> np.ndarray:
+def probe_block(n: int, settings: CissSettings, *, seed_offset: int = 0) -> np.ndarray:
     """Orthonormal complex probes from the fixed seed, generated exactly as the
     reference does (ciss.py:284-292): standard normals in float32, /sqrt(2),
-    reduced QR.  The block is k-independent, so it is built once per (n, L)."""
-    key = (int(n), settings.block_size, settings.random_seed)
+    reduced QR.  The block is k-independent, so it is built once per (n, L).
+    ``seed_offset = 1`` gives the independent block of 2L columns that
+    augments the filter passes (a pass subspace of about 1.5-2x the in-window
+    count keeps the subspace iteration contracting fast)."""
+    width = settings.block_size if seed_offset == 0 else 2 * settings.block_size
+    key = (int(n), width, settings.random_seed + seed_offset)
     if key not in _PROBE_CACHE:
-        rng = np.random.default_rng(settings.random_seed)
-        shape = (n, settings.block_size)
+        rng = np.random.default_rng(settings.random_seed + seed_offset)
+        shape = (n, width)
         v = rng.standard_normal(shape, dtype=np.float32) + 1j * rng.standard_normal(shape, dtype=np.float32)
         v = np.asarray(v / np.asarray(math.sqrt(2.0), dtype=np.float32), dtype=np.complex64)
         q, _ = np.linalg.qr(v, mode="reduced")
@@ -158,9 +162,14 @@ def band_solve(
         raise SolverError("CISS search dimension must be smaller than matrix dimension")
     started = perf_counter()
     probes = probe_block(native.n, settings)
+    augmentation = probe_block(native.n, settings, seed_offset=1) if settings.refinement_passes else None
     setup = perf_counter() - started
     req = N.new(N.BandRequest)
     keep: list[np.ndarray] = [probes]
+    if augmentation is not None:
+        keep.append(augmentation)
+        req.augmentation = N.ptr(augmentation, N.Complex64)
+        req.augmentation_width = augmentation.shape[1]
     if kpoints is not None:
         k = np.ascontiguousarray(np.asarray(kpoints, dtype=np.float64).reshape(-1, 3))
         keep.append(k)

@@ -318,7 +318,10 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
     } else if (!finite_c64(q->h_data, q->n_kpoints * b.nnz) || !finite_c64(q->s_data, q->n_kpoints * b.nnz)) {
         return fail(res, AOSX_NONFINITE_VALUES, 0, "H/S values contain non-finite entries");
     }
-    if (!finite_c64(q->probes, b.n * q->block_size))
+    if (q->augmentation && (q->augmentation_width < 32 || q->augmentation_width % 32 || q->augmentation_width > 224))
+        return fail(res, AOSX_INVALID_SETTINGS, 0, "augmentation_width must be a multiple of 32 in [32, 224]");
+    if (!finite_c64(q->probes, b.n * q->block_size) ||
+        (q->augmentation && !finite_c64(q->augmentation, b.n * q->augmentation_width)))
         return fail(res, AOSX_NONFINITE_RHS, 0, "probe block contains non-finite values");
     return guarded(res, [&]() -> int {
         aosx::CissParams p;
@@ -340,7 +343,8 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         p.conjugate_pairs = q->conjugate_pairs;
         aosx::CissOutput o;
         b.band(p, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),
-               reinterpret_cast<const float2 *>(q->s_data), reinterpret_cast<const float2 *>(q->probes), o);
+               reinterpret_cast<const float2 *>(q->s_data), reinterpret_cast<const float2 *>(q->probes),
+               reinterpret_cast<const float2 *>(q->augmentation), q->augmentation_width, o);
         for (int64_t k = 0; k < q->n_kpoints; ++k) {
             const auto &e = o.energies[k];
             q->energy_counts[k] = (int32_t)e.size();

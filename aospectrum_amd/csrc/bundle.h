@@ -109,7 +109,7 @@ struct Bundle {
                     const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
                     float2 *x_out, double *rel_residual);
     void band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
-              const float2 *probes, CissOutput &out);
+              const float2 *probes, const float2 *augmentation, int augmentation_width, CissOutput &out);
 };
 
 }  // namespace aosx

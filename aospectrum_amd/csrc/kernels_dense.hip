@@ -59,6 +59,20 @@ __global__ void k_moments(float2 *__restrict__ M, const float2 *__restrict__ X, 
     }
 }
 
+// dst[k][row][dcol0 + c] = src[k][row][scol0 + c] for c < ncols
+__global__ void k_copy_cols(float2 *__restrict__ dst, int dw, int dcol0, const float2 *__restrict__ src, int sw,
+                            int scol0, int ncols, int64_t n) {
+    const int k = blockIdx.y;
+    const float2 *s = src + (int64_t)k * n * sw;
+    float2 *d = dst + (int64_t)k * n * dw;
+    const int64_t total = n * ncols, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t row = e / ncols;
+        const int c = (int)(e % ncols);
+        d[row * dw + dcol0 + c] = s[row * sw + scol0 + c];
+    }
+}
+
 // Partial Gram: part[k][chunk][i][j] = sum_{rows of chunk} conj(A[row][i]) B[row][j]
 // over a 32x32 output tile (grid.x tiles, grid.y chunks, grid.z problems).
 __global__ void __launch_bounds__(256) k_gram_partial(c64 *__restrict__ part, const float2 *__restrict__ A,

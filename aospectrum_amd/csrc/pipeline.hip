@@ -9,13 +9,16 @@
 // and the maximum eigenpair residual (ciss.py:355-372).  Added here:
 //   * conjugate pairs: z_{N-1-j} = conj(z_j) and (H - conj(z) S) = (H - z S)^H,
 //     so one factorisation serves both shifts (the second through A^H = U^H L^H);
-//   * filter passes: the Ritz block of the previous pass is filtered again
-//     through the resident factorisations (moment 0 only) and projected anew,
-//     which squares the out-of-window leak of the contour filter.
+//   * filter passes: the Ritz block of the previous pass, augmented with the
+//     (re-)filtered probe block, is filtered again through the resident
+//     factorisations (moment 0 only) and projected anew, which squares the
+//     out-of-window leak of the contour filter.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "bundle.h"
@@ -385,6 +388,15 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     std::vector<double> w((size_t)nk * P);
     AOSX_HIP(hipMemcpyAsync(w.data(), d_w, w.size() * 8, hipMemcpyDeviceToHost, st));
     AOSX_HIP(hipStreamSynchronize(st));
+    if (std::getenv("AOSX_DEBUG")) {
+        for (int k = 0; k < nk; ++k) {
+            const double top = w[(size_t)k * P + P - 1];
+            std::fprintf(stderr, "AOSX_DEBUG|gram|k=%d|P=%d|lambda/top:", k, P);
+            for (int i = P - 1; i >= 0 && i >= P - 140; i -= 4)
+                std::fprintf(stderr, " %d:%.1e", P - 1 - i, w[(size_t)k * P + i] / top);
+            std::fprintf(stderr, "\n");
+        }
+    }
     // rank truncation (ciss.py:107-126)
     out.rank.assign(nk, 0);
     std::vector<std::vector<int32_t>> keep(nk);
@@ -562,7 +574,7 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
 }
 
 void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
-                  const float2 *probes, CissOutput &out) {
+                  const float2 *probes, const float2 *augmentation, int augmentation_width, CissOutput &out) {
     AOSX_HIP(hipSetDevice(device));
     const double t_start = now();
     const int N = p.integration_points, L = p.block_size, Ms = p.moment_size, P = L * Ms;
@@ -722,12 +734,35 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
         compress_and_project(*this, Kc, d_M, P, d_H, d_S, p, sub, out.stages,
                              p.refinement_passes > 0 ? ritz_names[0] : nullptr);
         std::vector<int> rank0 = sub.rank;
+        // filter passes: the block [Ritz vectors | augmentation] is filtered
+        // again through the resident factorisations.  The augmentation starts
+        // as a second independent probe block and is itself re-filtered every
+        // pass: it supplies in-window directions that the monomial moment
+        // basis loses to the rank truncation when the in-window states bunch
+        // at one end of the window (their moments become nearly collinear).
+        float2 *d_aug = nullptr;
+        const int A = augmentation ? augmentation_width : 0;
+        if (p.refinement_passes > 0 && augmentation) {
+            d_aug = buf<float2>("p_aug", (size_t)Kc * n * A);
+            for (int k = 0; k < Kc; ++k)
+                AOSX_HIP(hipMemcpyAsync(d_aug + (size_t)k * n * A, augmentation, (size_t)n * A * sizeof(float2),
+                                        hipMemcpyHostToDevice, stream));
+        }
         for (int pass = 0; pass < p.refinement_passes; ++pass) {
             double tp = now();
-            const int W2 = sub.rmax;  // multiple of 32
-            // RHS2 = S * Ritz block of the previous projection
+            const int R = sub.rmax;  // multiple of 32
+            const bool augment = d_aug && R + A <= 256;
+            const int W2 = augment ? R + A : R;
+            float2 *d_blk = buf<float2>("p_blk", (size_t)Kc * n * W2);
+            k_copy_cols<<<dim3(blocks_for(n * R, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, 0, sub.ritz, R, 0, R, n);
+            AOSX_LAUNCHED();
+            if (augment) {
+                k_copy_cols<<<dim3(blocks_for(n * A, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, R, d_aug, A, 0, A,
+                                                                                       n);
+                AOSX_LAUNCHED();
+            }
             float2 *d_rhs2 = buf<float2>("p_rhs", (size_t)Kc * n * W2);
-            k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs2, d_S, d_iota, sub.ritz, d_iota, W2,
+            k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs2, d_S, d_iota, d_blk, d_iota, W2,
                                                                         d_indptr, d_indices, n, nnz);
             AOSX_LAUNCHED();
             float2 *d_Y = buf<float2>("p_Y", (size_t)Kc * n * W2);
@@ -748,12 +783,21 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                     accumulate(gr, true, d_Y, W2, 1);
                 }
             }
+            if (augment) {
+                k_copy_cols<<<dim3(blocks_for(n * A, 256, 4096), Kc), 256, 0, stream>>>(d_aug, A, 0, d_Y, W2, R, A,
+                                                                                       n);
+                AOSX_LAUNCHED();
+            }
             AOSX_HIP(hipStreamSynchronize(stream));
             out.stages[9] += now() - tp;
             const bool more = pass + 1 < p.refinement_passes;
             Subspace next;
             compress_and_project(*this, Kc, d_Y, W2, d_H, d_S, p, next, out.stages,
                                  more ? ritz_names[(pass + 1) & 1] : nullptr);
+            if (std::getenv("AOSX_DEBUG"))
+                for (int k = 0; k < Kc; ++k)
+                    std::fprintf(stderr, "AOSX_DEBUG|pass %d|k=%d|W=%d|rank=%d|count=%zu|residual=%.3e\n", pass + 1, k,
+                                 W2, next.rank[k], next.energies[k].size(), next.residual[k]);
             sub = next;
         }
         for (int k = 0; k < Kc; ++k) {

@@ -229,6 +229,11 @@ typedef struct {
     double rank_relative_tolerance;
     double spurious_relative_threshold;
     const aosx_complex64 *probes;  /* [n][block_size] orthonormal probe block      */
+    const aosx_complex64 *augmentation; /* [n][augmentation_width] independent probe
+                                      block joined to the Ritz block of the first
+                                      filter pass, or NULL                      */
+    int32_t augmentation_width;    /* columns of `augmentation` (multiple of 32)   */
+    int32_t reserved_aug;
     /* MI355X-specific execution knobs */
     int32_t precision;             /* AOSX_PRECISION_*                             */
     int32_t refinement_steps;      /* IR steps per solve (*_IR precisions)         */

@@ -147,3 +147,58 @@ def test_band_solve_matches_oracle_ciss(small2d, small2d_pattern, native_small):
     assert abs(r.subspace_rank - ref.rank) <= 1
     # same order of residual as the reference algorithm (both ~1e-3 eV here)
     assert r.maximum_residual_ev < 5 * ref.maximum_residual_ev + 1e-4
+
+
+@pytest.fixture(scope="module")
+def proxy3d():
+    """216-atom periodic 3D bundle with 77 states in [-1.02, 1.02] eV -- more
+    than two moment blocks of 32 probes can hold, so moments 2 and 3 matter."""
+    return synth.generate(synth.scaled_spec("bulk3d", grid=(6, 6, 6), cell=(14.4, 14.4, 14.4), gap=1.8))
+
+
+@pytest.mark.parametrize("label", ["plain", "mixed"])
+def test_band_solve_needs_all_moments(proxy3d, label):
+    pat = bloch.Pattern.build(proxy3d.operators, proxy3d.basis.atom_offsets, proxy3d.n_orbitals)
+    k = np.array([0.0, 0.2, 0.0])
+    interval = EnergyInterval(-1.02, 1.02)
+    h, s = pat.assemble(proxy3d.operators, k)
+    ref = dense.window_spectrum(pat.csr(h).toarray(), pat.csr(s).toarray(), interval.lower_ev, interval.upper_ev)
+    settings = {
+        "plain": CissSettings(precision="fp32_ir", refinement_steps=2, refinement_passes=2),
+        "mixed": CissSettings(contour_batch_size=16, conjugate_pairs=True, precision="bf16x3_ir",
+                              refinement_steps=2, refinement_passes=2),
+    }[label]
+    say("proxy3d", label, "oracle count", ref.size)
+    nb = NativeBundle(proxy3d, device=0)
+    try:
+        (r,), _ = band_solve(nb, settings, interval, kpoints=k[None, :])
+    finally:
+        nb.close()
+    say(f"  count={r.energies_ev.size} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e}")
+    assert r.subspace_rank >= ref.size
+    assert r.energies_ev.size == ref.size
+    assert np.max(np.abs(np.sort(r.energies_ev) - ref)) < 1e-4
+    assert r.maximum_residual_ev <= 1e-4
+
+
+def test_bulk3d_matches_golden_inertia_counts():
+    """13,000-orbital bulk: in-window counts equal the FP64 Sylvester-inertia
+    counts committed in tests/golden/bulk3d_counts.json, residual <= 1e-4 eV."""
+    import json
+    from pathlib import Path
+
+    golden = json.loads((Path(__file__).parent / "golden" / "bulk3d_counts.json").read_text())
+    bundle = synth.generate(synth.SPECS["bulk3d"])
+    lo, hi = golden["interval_ev"]
+    settings = CissSettings(contour_batch_size=16, k_batch=3, conjugate_pairs=True, precision="bf16x3_ir",
+                            refinement_steps=2, refinement_passes=2)
+    say("bulk3d golden counts", golden["counts"])
+    nb = NativeBundle(bundle, device=0)
+    try:
+        results, _ = band_solve(nb, settings, EnergyInterval(lo, hi), kpoints=np.asarray(golden["kpoints"]))
+    finally:
+        nb.close()
+    for r, count in zip(results, golden["counts"]):
+        say(f"  count={r.energies_ev.size} golden={count} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e}")
+        assert r.energies_ev.size == count
+        assert r.maximum_residual_ev <= 1e-4
