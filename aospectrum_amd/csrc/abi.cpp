@@ -162,7 +162,7 @@ int aosx_bundle_create_v1(const aosx_bundle_request_v1 *q, aosx_bundle_result_v1
                 return fail(res, AOSX_NO_DEVICE, 0, "HIP device %d is not available", q->device);
             }
             b->device = q->device;
-            b->upload(q->h_values, q->s_values, q->value_kind, n_values);
+            b->upload(q->h_values, q->s_values, q->value_kind, n_values, q->atom_offsets, q->n_atoms);
             res->resident_device_bytes = b->resident_bytes;
         }
         res->upload_seconds = now() - t3;
@@ -262,7 +262,7 @@ int aosx_shift_solve_v1(aosx_bundle *bundle, const aosx_shift_request_v1 *q, aos
         const bool ir = q->precision != AOSX_PRECISION_FP32;
         double rel = 0.0;
         b.solve_full(nb, W, d_rhs, kidx, d_h, d_s, kidx, z, tiles, dinv, false, ir ? q->refinement_steps : 0, d_x,
-                     &rel);
+                     &rel, q->precision);
         AOSX_HIP(hipStreamSynchronize(b.stream));
         double t3 = now();
         std::vector<float2> x((size_t)nb * n * W);

@@ -78,6 +78,9 @@ struct Bundle {
     int32_t *d_source = nullptr, *d_group = nullptr, *d_starts = nullptr, *d_mirror = nullptr;
     int32_t *d_rows = nullptr, *d_indices = nullptr, *d_indptr = nullptr, *d_shifts = nullptr;
     int32_t *d_iperm = nullptr, *d_perm = nullptr, *d_tile_id = nullptr, *d_next = nullptr, *d_prev = nullptr;
+    int32_t *d_atom_off = nullptr;   // [n_atoms + 1] orbital offsets (atom-block SpMM)
+    int64_t n_atoms = 0;
+    int max_atom = 0;                // largest atom, in orbitals
     uint64_t resident_bytes = 0;
     uint64_t peak_bytes = 0;
     KernelStats kstats;
@@ -98,16 +101,25 @@ struct Bundle {
     size_t factor_floats() const { return (size_t)ts.n_tiles * 2 * 64 * 64; }
 
     void zero(void *p, size_t bytes);  // bytes: multiple of 16
-    void upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values);
+    // Y[j] = A[kidx[j]] X[xidx[j]] (n x w complex64) for j < nmat
+    void spmm(float2 *y, const float2 *a, const int32_t *d_kidx, const float2 *x, const int32_t *d_xidx, int w,
+              int nmat);
+    // r[m] = b[b_index[m]] - (H_k - z_m S_k) x[m] in FP64, norms into sumsq/bsq
+    void residual(float2 *r, const float2 *b, const int32_t *d_bidx, const float2 *x, int w, const float2 *d_h,
+                  const float2 *d_s, const int32_t *d_kidx, const float2 *d_z, int nmat, double *sumsq, double *bsq);
+    void upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values,
+                const int64_t *atom_offsets, int64_t atoms);
     // pipeline stages
     void assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *d_s);
     void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
                 float *tiles, float *dinv, int *d_replaced);
-    void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans);
+    // precision selects the MFMA form of the solve products (BF16 split for
+    // the BF16 precisions, FP32 otherwise)
+    void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans, int precision);
     void solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
                     const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                     const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
-                    float2 *x_out, double *rel_residual);
+                    float2 *x_out, double *rel_residual, int precision);
     void band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
               const float2 *probes, const float2 *augmentation, int augmentation_width, CissOutput &out);
 };

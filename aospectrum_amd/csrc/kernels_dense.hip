@@ -39,6 +39,107 @@ __global__ void __launch_bounds__(256) k_spmm(float2 *__restrict__ y, const floa
     }
 }
 
+// Atom-block CSR x tall-skinny product.  Every orbital row of an atom has
+// the same column list (the pattern is a union of full atom blocks), so one
+// workgroup takes one atom's rows and reads each needed row of x once for all
+// of them -- an atom's worth (13x) less gather traffic than row-wise SpMM.
+// Values are staged through LDS in chunks of 64 columns; accumulation is FP64.
+//   RESIDUAL: out[m] = b[b_index[m]] - (H_k - z_m S_k) x[m], with ||out||^2 and
+//             ||b||^2 added to sumsq[m], bsq[m] (FP64 iterative refinement);
+//   else    : out[m] = A_k x[x_index[m]] with A = h (H or S).
+// Wave q of the block owns rows q, q+4, ... of the atom (at most 8 rows each,
+// so atoms of up to 32 orbitals); lanes own columns.
+template <bool RESIDUAL>
+__global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, const float2 *__restrict__ b,
+                                                  const int32_t *__restrict__ b_index, const float2 *__restrict__ x,
+                                                  const int32_t *__restrict__ x_index, int w,
+                                                  const float2 *__restrict__ h, const float2 *__restrict__ s,
+                                                  const int32_t *__restrict__ kidx, const float2 *__restrict__ z,
+                                                  const int32_t *__restrict__ indptr,
+                                                  const int32_t *__restrict__ indices,
+                                                  const int32_t *__restrict__ atom_off, int64_t n, int64_t nnz,
+                                                  double *__restrict__ sumsq, double *__restrict__ bsq) {
+    constexpr int CH = 64, MAXR = 32;
+    __shared__ double2 vals[CH][MAXR];
+    __shared__ int32_t cidx[CH];
+    const int a = blockIdx.x, m = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = atom_off[a], na = atom_off[a + 1] - r0;
+    const int p0 = indptr[r0], len = indptr[r0 + 1] - p0;
+    const int k = kidx[m];
+    const float2 *hk = h + (int64_t)k * nnz;
+    const float2 *sk = RESIDUAL ? s + (int64_t)k * nnz : nullptr;
+    const double zr = RESIDUAL ? (double)z[m].x : 0.0, zi = RESIDUAL ? (double)z[m].y : 0.0;
+    const float2 *xm = x + (int64_t)(RESIDUAL ? m : x_index[m]) * n * w;
+    float2 *om = out + (int64_t)m * n * w;
+    double local = 0.0, blocal = 0.0;
+    for (int c0 = 0; c0 < w; c0 += 64) {
+        const int c = c0 + lane;
+        double accr[8], acci[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) accr[q] = acci[q] = 0.0;
+        for (int q0 = 0; q0 < len; q0 += CH) {
+            const int cnt = min(CH, len - q0);
+            __syncthreads();
+            for (int e = tid; e < cnt * na; e += 256) {
+                const int pp = e / na, r = e % na;
+                const int64_t pos = indptr[r0 + r] + q0 + pp;
+                const float2 hv = hk[pos];
+                double vr = hv.x, vi = hv.y;
+                if (RESIDUAL) {
+                    const float2 sv = sk[pos];
+                    vr -= zr * sv.x - zi * sv.y;
+                    vi -= zr * sv.y + zi * sv.x;
+                }
+                vals[pp][r] = make_double2(vr, vi);
+            }
+            for (int e = tid; e < cnt; e += 256) cidx[e] = indices[p0 + q0 + e];
+            __syncthreads();
+            if (c < w) {
+                for (int pp = 0; pp < cnt; ++pp) {
+                    const float2 xv = xm[(int64_t)cidx[pp] * w + c];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int r = wave + 4 * q;
+                        if (r < na) {
+                            const double2 v = vals[pp][r];
+                            accr[q] += v.x * xv.x - v.y * xv.y;
+                            acci[q] += v.x * xv.y + v.y * xv.x;
+                        }
+                    }
+                }
+            }
+        }
+        if (c < w) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int r = wave + 4 * q;
+                if (r >= na) continue;
+                const int64_t row = r0 + r;
+                if (RESIDUAL) {
+                    const float2 bv = b[(int64_t)b_index[m] * n * w + row * w + c];
+                    const double rr = bv.x - accr[q], ri = bv.y - acci[q];
+                    om[row * w + c] = make_float2((float)rr, (float)ri);
+                    local += rr * rr + ri * ri;
+                    blocal += (double)bv.x * bv.x + (double)bv.y * bv.y;
+                } else {
+                    om[row * w + c] = make_float2((float)accr[q], (float)acci[q]);
+                }
+            }
+        }
+    }
+    if (RESIDUAL) {
+        for (int off = 32; off > 0; off >>= 1) {
+            local += __shfl_down(local, off);
+            blocal += __shfl_down(blocal, off);
+        }
+        if (lane == 0) {
+            atomicAdd(sumsq + m, local);
+            atomicAdd(bsq + m, blocal);
+        }
+    }
+}
+
 // M[row][m*L + c] += sum_b coef[b*msize + m] * X_b[row][c]   (b over nb solutions)
 __global__ void k_moments(float2 *__restrict__ M, const float2 *__restrict__ X, int nb, int L, int msize,
                           const float2 *__restrict__ coef, int64_t n) {

@@ -29,7 +29,10 @@ __device__ __forceinline__ void stage_a_ctrans(const float *__restrict__ g, floa
 }
 
 // Y_k <- D_k Y_k for all chunks; D_k = inv(L_kk) (which 0) or inv(U_kk)
-// (which 1), conjugate-transposed when ctrans.
+// (which 1), conjugate-transposed when ctrans.  MODE 0 multiplies with FP32
+// MFMA, MODE 1 with the three-product BF16 split (refinement on FP64
+// residuals removes the difference).
+template <int MODE>
 __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                    const float *__restrict__ dinv, int nt, int k, int which,
                                                    int ctrans) {
@@ -45,13 +48,20 @@ __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64
         stage_b(yk, Bre, Bim, tid);
         __syncthreads();
         floatx16 cre = {}, cim = {};
-        mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        if (MODE == 0) {
+            mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        } else {
+            mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+            cre = -cre;
+            cim = -cim;
+        }
         store_acc(yk, cre, cim, wave, lane);
     }
 }
 
 // Y_i -= T Y_k for i = list[blockIdx.x], all chunks; T is tile (i,k) or,
-// with ctrans, the conjugate transpose of tile (k,i).
+// with ctrans, the conjugate transpose of tile (k,i).  MODE as k_solve_diag.
+template <int MODE>
 __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                      const float *__restrict__ tiles, int64_t mat_stride,
                                                      const int32_t *__restrict__ tile_id, int nt, int k,
@@ -72,7 +82,8 @@ __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int
         floatx16 cre, cim;
         load_acc(yi, cre, cim, wave, lane);
         __syncthreads();
-        mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        if (MODE == 0) mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        else mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
         store_acc(yi, cre, cim, wave, lane);
     }
 }

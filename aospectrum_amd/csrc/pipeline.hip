@@ -69,7 +69,8 @@ Bundle::~Bundle() {
         release_workspace();
         for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
-                        (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev})
+                        (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
+                        (void *)d_atom_off})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -129,7 +130,8 @@ static T *upload_vec(const std::vector<T> &v, uint64_t &counter) {
     return p;
 }
 
-void Bundle::upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values) {
+void Bundle::upload(const void *h_values, const void *s_values, int value_kind, int64_t n_values,
+                    const int64_t *atom_offsets, int64_t atoms) {
     AOSX_HIP(hipSetDevice(device));
     AOSX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     std::vector<float2> tmp(n_values);
@@ -161,7 +163,39 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     d_tile_id = upload_vec(ts.tile_id, resident_bytes);
     d_next = upload_vec(ts.next_idx, resident_bytes);
     d_prev = upload_vec(ts.prev_idx, resident_bytes);
+    n_atoms = atoms;
+    std::vector<int32_t> off(atoms + 1);
+    max_atom = 0;
+    for (int64_t a = 0; a <= atoms; ++a) off[a] = (int32_t)atom_offsets[a];
+    for (int64_t a = 0; a < atoms; ++a) max_atom = std::max<int>(max_atom, off[a + 1] - off[a]);
+    d_atom_off = upload_vec(off, resident_bytes);
     peak_bytes = resident_bytes;
+}
+
+void Bundle::spmm(float2 *y, const float2 *a, const int32_t *d_kidx, const float2 *x, const int32_t *d_xidx, int w,
+                  int nmat) {
+    if (max_atom <= 32) {
+        k_atom_spmm<false><<<dim3((unsigned)n_atoms, nmat), 256, 0, stream>>>(
+            y, nullptr, nullptr, x, d_xidx, w, a, nullptr, d_kidx, nullptr, d_indptr, d_indices, d_atom_off, n, nnz,
+            nullptr, nullptr);
+    } else {
+        k_spmm<<<dim3(blocks_for(n, 4, 2048), nmat), 256, 0, stream>>>(y, a, d_kidx, x, d_xidx, w, d_indptr,
+                                                                       d_indices, n, nnz);
+    }
+    AOSX_LAUNCHED();
+}
+
+void Bundle::residual(float2 *r, const float2 *b, const int32_t *d_bidx, const float2 *x, int w, const float2 *d_h,
+                      const float2 *d_s, const int32_t *d_kidx, const float2 *d_z, int nmat, double *sumsq,
+                      double *bsq) {
+    if (max_atom <= 32) {
+        k_atom_spmm<true><<<dim3((unsigned)n_atoms, nmat), 256, 0, stream>>>(
+            r, b, d_bidx, x, nullptr, w, d_h, d_s, d_kidx, d_z, d_indptr, d_indices, d_atom_off, n, nnz, sumsq, bsq);
+    } else {
+        k_residual_shifted<<<dim3(blocks_for(n, 4, 2048), nmat), 256, 0, stream>>>(
+            r, b, d_bidx, x, w, d_h, d_s, d_kidx, d_z, d_indptr, d_indices, n, nnz, sumsq, bsq);
+    }
+    AOSX_LAUNCHED();
 }
 
 // ------------------------------------------------------------------ assembly
@@ -243,29 +277,32 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
 
 // ------------------------------------------------------------------- solves
 // y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
-void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans) {
+void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision) {
+    const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
+    auto diag = split ? k_solve_diag<1> : k_solve_diag<0>;
+    auto update = split ? k_solve_update<1> : k_solve_update<0>;
     float *y = reinterpret_cast<float *>(y2);
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
     EventTimer tm(kstats.enabled, stream);
     for (int k = 0; k < nt; ++k) {
-        k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
+        diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
         AOSX_LAUNCHED();
         const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
         if (s) {
-            k_solve_update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+            update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                      d_next + ts.next_ptr[k], ctrans ? 1 : 0);
             AOSX_LAUNCHED();
         }
     }
     for (int k = nt - 1; k >= 0; --k) {
-        k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
+        diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
         AOSX_LAUNCHED();
         const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
         if (s) {
-            k_solve_update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
+            update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                      d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
             AOSX_LAUNCHED();
         }
     }
@@ -282,7 +319,7 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
 void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
                         const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                         const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
-                        float2 *x_out, double *rel_residual) {
+                        float2 *x_out, double *rel_residual, int precision) {
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t padded = (int64_t)nt * 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
@@ -302,21 +339,18 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     float2 *y2 = reinterpret_cast<float2 *>(y);
     k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, d_rhs, W, d_ri, d_perm, n, padded);
     AOSX_LAUNCHED();
-    solve(nmat, W, y2, tiles, dinv, ctrans);
+    solve(nmat, W, y2, tiles, dinv, ctrans, precision);
     k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(x_out, W, y, ystride, nchunk, d_perm, n);
     AOSX_LAUNCHED();
     if (ir_steps <= 0 && !rel_residual) return;
     float2 *r = buf<float2>("s_r", (size_t)nmat * n * W);
     double *d_norm = buf<double>("s_norm", 2 * (size_t)nmat);
-    const int rb = blocks_for(n, 4, 2048);
     for (int step = 0; step <= ir_steps; ++step) {
         const bool last = step == ir_steps;
         if (last && !rel_residual) break;
         k_fill_zero<<<1, 64, 0, stream>>>(reinterpret_cast<float4 *>(d_norm), (int64_t)nmat);
         AOSX_LAUNCHED();
-        k_residual_shifted<<<dim3(rb, nmat), 256, 0, stream>>>(r, d_rhs, d_ri, x_out, W, d_h, d_s, d_ki, d_z,
-                                                               d_indptr, d_indices, n, nnz, d_norm, d_norm + nmat);
-        AOSX_LAUNCHED();
+        residual(r, d_rhs, d_ri, x_out, W, d_h, d_s, d_ki, d_z, nmat, d_norm, d_norm + nmat);
         if (last) {
             std::vector<double> hn(2 * (size_t)nmat);
             AOSX_HIP(hipMemcpyAsync(hn.data(), d_norm, hn.size() * 8, hipMemcpyDeviceToHost, stream));
@@ -329,7 +363,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
         }
         k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, r, W, d_id, d_perm, n, padded);
         AOSX_LAUNCHED();
-        solve(nmat, W, y2, tiles, dinv, ctrans);
+        solve(nmat, W, y2, tiles, dinv, ctrans, precision);
         k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(r, W, y, ystride, nchunk, d_perm, n);
         AOSX_LAUNCHED();
         k_axpy<<<blocks_for((int64_t)nmat * n * W, 256, 8192), 256, 0, stream>>>(x_out, r, (int64_t)nmat * n * W);
@@ -360,9 +394,7 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     AOSX_HIP(hipMemcpyAsync(d_iota, iota_k.data(), nk * 4, hipMemcpyHostToDevice, st));
     // SM = S M, G = M^H S M (FP64, hermitised)
     float2 *d_SM = B.buf<float2>("c_SM", (size_t)nk * n * P);
-    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_SM, d_S, d_iota, d_M, d_iota, P, B.d_indptr,
-                                                             B.d_indices, n, nnz);
-    AOSX_LAUNCHED();
+    B.spmm(d_SM, d_S, d_iota, d_M, d_iota, P, nk);
     auto gram = [&](c64 *G, int ldg, const float2 *A, const float2 *Bm, int Pc, int64_t stride_a, int herm) {
         const int chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, n / 256));
         const int rpc = (int)(((n + chunks - 1) / chunks + 31) / 32 * 32);
@@ -450,12 +482,8 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     // Rayleigh-Ritz: Hh = Q^H H Q, Sh = Q^H S Q (FP64, hermitised)
     float2 *d_HQ = B.buf<float2>("c_HQ", (size_t)nk * n * R);
     float2 *d_SQ = B.buf<float2>("c_SQ", (size_t)nk * n * R);
-    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_HQ, d_H, d_iota, d_Q, d_iota, R, B.d_indptr,
-                                                             B.d_indices, n, nnz);
-    AOSX_LAUNCHED();
-    k_spmm<<<dim3(blocks_for(n, 4, 2048), nk), 256, 0, st>>>(d_SQ, d_S, d_iota, d_Q, d_iota, R, B.d_indptr,
-                                                             B.d_indices, n, nnz);
-    AOSX_LAUNCHED();
+    B.spmm(d_HQ, d_H, d_iota, d_Q, d_iota, R, nk);
+    B.spmm(d_SQ, d_S, d_iota, d_Q, d_iota, R, nk);
     const int64_t RR = (int64_t)R * R;
     c64 *d_Hh = B.buf<c64>("c_Hh", (size_t)nk * RR);
     c64 *d_Sh = B.buf<c64>("c_Sh", (size_t)nk * RR);
@@ -622,6 +650,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     const bool resident = p.refinement_passes > 0;
     const bool ir = p.precision != AOSX_PRECISION_FP32;
     const int ir_steps = ir ? std::max(0, p.refinement_steps) : 0;
+    // refinement schedule: with filter passes ahead, the moment pass and the
+    // intermediate passes only need a subspace, not converged solves -- one
+    // step there, the full count on the last filter application
+    const int ir_first = p.refinement_passes > 0 ? std::min(ir_steps, 1) : ir_steps;
     int *d_repl = buf<int>("replaced", 65536);
     out.stages[1] += now() - t_start;
     for (int64_t kb = 0; kb < nk_total; kb += K) {
@@ -645,9 +677,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
         AOSX_HIP(hipMemcpyAsync(d_iota, iota.data(), Kc * 4, hipMemcpyHostToDevice, stream));
         AOSX_HIP(hipMemcpyAsync(d_zero, zeros.data(), Kc * 4, hipMemcpyHostToDevice, stream));
         float2 *d_rhs = buf<float2>("b_rhs", (size_t)Kc * n * L);
-        k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs, d_S, d_iota, d_V, d_zero, L, d_indptr,
-                                                                    d_indices, n, nnz);
-        AOSX_LAUNCHED();
+        spmm(d_rhs, d_S, d_iota, d_V, d_zero, L, Kc);
         float2 *d_M = buf<float2>("b_M", (size_t)Kc * n * P);
         const int64_t m4 = (int64_t)Kc * n * P * 8 / 16;
         k_fill_zero<<<blocks_for(m4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(d_M), m4);
@@ -716,10 +746,12 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             double tb = now();
             out.stages[2] += tb - ta;
             out.factorizations += gsz;
-            solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_steps, d_X, nullptr);
+            solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_first, d_X, nullptr,
+                       p.precision);
             accumulate(gr, false, d_M, L, Ms);
             if (p.conjugate_pairs) {
-                solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_steps, d_X, nullptr);
+                solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_first, d_X, nullptr,
+                           p.precision);
                 accumulate(gr, true, d_M, L, Ms);
             }
             AOSX_HIP(hipStreamSynchronize(stream));
@@ -753,6 +785,8 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             const int R = sub.rmax;  // multiple of 32
             const bool augment = d_aug && R + A <= 256;
             const int W2 = augment ? R + A : R;
+            const bool last_pass = pass + 1 == p.refinement_passes;
+            const int ir_pass = last_pass ? ir_steps : std::min(ir_steps, 1);
             float2 *d_blk = buf<float2>("p_blk", (size_t)Kc * n * W2);
             k_copy_cols<<<dim3(blocks_for(n * R, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, 0, sub.ritz, R, 0, R, n);
             AOSX_LAUNCHED();
@@ -762,9 +796,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 AOSX_LAUNCHED();
             }
             float2 *d_rhs2 = buf<float2>("p_rhs", (size_t)Kc * n * W2);
-            k_spmm<<<dim3(blocks_for(n, 4, 2048), Kc), 256, 0, stream>>>(d_rhs2, d_S, d_iota, d_blk, d_iota, W2,
-                                                                        d_indptr, d_indices, n, nnz);
-            AOSX_LAUNCHED();
+            spmm(d_rhs2, d_S, d_iota, d_blk, d_iota, W2, Kc);
             float2 *d_Y = buf<float2>("p_Y", (size_t)Kc * n * W2);
             const int64_t y4 = (int64_t)Kc * n * W2 * 8 / 16;
             k_fill_zero<<<blocks_for(y4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(d_Y), y4);
@@ -775,11 +807,12 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 const int gsz = (int)gr.kidx.size();
                 float *tl = d_tiles + (size_t)gr.slot0 * ff;
                 float *di = d_dinv + (size_t)gr.slot0 * dif;
-                solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_steps, d_X, nullptr);
+                solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_pass, d_X, nullptr,
+                           p.precision);
                 accumulate(gr, false, d_Y, W2, 1);
                 if (p.conjugate_pairs) {
-                    solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_steps, d_X,
-                               nullptr);
+                    solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_pass, d_X,
+                               nullptr, p.precision);
                     accumulate(gr, true, d_Y, W2, 1);
                 }
             }
