@@ -167,6 +167,7 @@ class BandRequest(C.Structure):
         ("s_data", _C64P),
         ("lower_ev", C.c_double),
         ("upper_ev", C.c_double),
+        ("intervals", _F64P),
         ("integration_points", C.c_int32),
         ("contour_batch_size", C.c_int32),
         ("block_size", C.c_int32),
@@ -278,6 +279,7 @@ def load() -> C.CDLL:
         "aosx_shift_solve_v1": [_P, C.POINTER(ShiftRequest), C.POINTER(ShiftResult)],
         "aosx_band_solve_v1": [_P, C.POINTER(BandRequest), C.POINTER(BandResult)],
         "aosx_kernel_stats_read_v1": [_P, C.c_int32, C.POINTER(KernelStats)],
+        "aosx_bundle_trim_v1": [_P, C.POINTER(C.c_uint64)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -296,6 +298,7 @@ EXPORTED_SYMBOLS = (
     "aosx_band_solve_v1",
     "aosx_kernel_stats_read_v1",
     "aosx_struct_sizes_v1",
+    "aosx_bundle_trim_v1",
 )
 
 

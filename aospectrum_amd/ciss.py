@@ -152,9 +152,12 @@ def band_solve(
     kpoints: np.ndarray | None = None,
     h_data: np.ndarray | None = None,
     s_data: np.ndarray | None = None,
+    intervals: np.ndarray | None = None,
 ) -> tuple[list[CissResult], dict[str, Any]]:
     """Run the CISS Band path for a batch of k points (on-device assembly) or
-    of explicit H/S value arrays on the bundle's pattern."""
+    of explicit H/S value arrays on the bundle's pattern.  ``intervals``
+    ([nk, 2], absolute eV) gives every k point its own window; otherwise all
+    use ``interval``."""
     lib = N.load()
     if (kpoints is None) == (h_data is None):
         raise InputError("give either k points or explicit H/S values")
@@ -182,6 +185,12 @@ def band_solve(
         nk = h.shape[0]
         req.h_data = N.ptr(h, N.Complex64)
         req.s_data = N.ptr(s, N.Complex64)
+    if intervals is not None:
+        iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64).reshape(-1, 2))
+        if iv.shape[0] != nk:
+            raise InputError("one interval per k point is required")
+        keep.append(iv)
+        req.intervals = N.ptr(iv, C.c_double)
     cap = settings.search_dimension
     energies = np.zeros((nk, cap), dtype=np.float32)
     counts = np.zeros(nk, dtype=np.int32)

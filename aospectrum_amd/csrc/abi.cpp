@@ -190,6 +190,21 @@ int aosx_bundle_destroy_v1(aosx_bundle *bundle) {
     return 0;
 }
 
+int aosx_bundle_trim_v1(aosx_bundle *bundle, uint64_t *resident_bytes) {
+    if (!bundle) return 1;
+    Bundle &b = *bundle->impl;
+    if (b.device >= 0) {
+        if (hipSetDevice(b.device) != hipSuccess || (b.stream && hipStreamSynchronize(b.stream) != hipSuccess)) {
+            (void)hipGetLastError();
+            return 1;
+        }
+        b.release_workspace();
+        b.peak_bytes = b.resident_bytes;
+    }
+    if (resident_bytes) *resident_bytes = b.resident_bytes;
+    return 0;
+}
+
 int aosx_assemble_v1(aosx_bundle *bundle, const aosx_assemble_request_v1 *q, aosx_status_v1 *res) {
     if (!q || !res) return 1;
     begin(res);
@@ -308,6 +323,12 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         return fail(res, AOSX_INVALID_SETTINGS, 0,
                     "CISS sizes must be positive, contour_batch_size must divide integration_points, "
                     "block_size must be a multiple of 32 and block_size*moment_size at most 256");
+    if (q->intervals)
+        for (int64_t k = 0; k < q->n_kpoints; ++k)
+            if (!(q->intervals[2 * k] < q->intervals[2 * k + 1]) || !std::isfinite(q->intervals[2 * k]) ||
+                !std::isfinite(q->intervals[2 * k + 1]))
+                return fail(res, AOSX_INVALID_SETTINGS, 0, "per-k interval %lld requires lower < upper",
+                            (long long)k);
     if ((int64_t)q->block_size * q->moment_size >= b.n)
         return fail(res, AOSX_SEARCH_DIMENSION, 0, "CISS search dimension must be smaller than matrix dimension");
     if (q->energy_capacity < q->block_size * q->moment_size)
@@ -341,6 +362,7 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         p.refinement_passes = q->refinement_passes;
         p.k_batch = q->k_batch;
         p.conjugate_pairs = q->conjugate_pairs;
+        p.intervals = q->intervals;
         aosx::CissOutput o;
         b.band(p, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),
                reinterpret_cast<const float2 *>(q->s_data), reinterpret_cast<const float2 *>(q->probes),

@@ -383,8 +383,8 @@ struct Subspace {
 }  // namespace
 
 static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, const float2 *d_H,
-                                 const float2 *d_S, const CissParams &p, Subspace &out, double *stage,
-                                 const char *ritz_name) {
+                                 const float2 *d_S, const CissParams &p, const double *lower, const double *upper,
+                                 Subspace &out, double *stage, const char *ritz_name) {
     const int64_t n = B.n, nnz = B.nnz;
     hipStream_t st = B.stream;
     double t0 = now();
@@ -550,7 +550,7 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
         for (int j = 0; j < out.rank[k]; ++j) {
             const double sc = score[(size_t)k * R + j];
             const float e32 = (float)E[(size_t)k * R + j];
-            if (std::isfinite(sc) && sc >= p.spurious * top && (double)e32 > p.lower && (double)e32 < p.upper) {
+            if (std::isfinite(sc) && sc >= p.spurious * top && (double)e32 > lower[k] && (double)e32 < upper[k]) {
                 sel[k].push_back(j);
                 out.energies[k].push_back(e32);
             }
@@ -608,34 +608,55 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     const int N = p.integration_points, L = p.block_size, Ms = p.moment_size, P = L * Ms;
     if (p.conjugate_pairs && (N % 2)) throw Error(AOSX_INVALID_SETTINGS, "conjugate pairs need an even integration_points");
     if (L % 32) throw Error(AOSX_INVALID_SETTINGS, "block_size must be a multiple of 32 on this device path");
-    // contour (ciss.py:81-104), FP64 then complex64
-    const double lo = p.lower - 2.0 * p.guard, hi = p.upper + 2.0 * p.guard;
-    const double cen = 0.5 * (lo + hi), rh = 0.5 * (hi - lo), rv = rh * p.vscale;
-    std::vector<std::complex<double>> zc(N), tc(N), wc(N);
-    for (int j = 0; j < N; ++j) {
-        const double th = 2.0 * M_PI * (j + 0.5) / N;
-        tc[j] = {std::cos(th), p.vscale * std::sin(th)};
-        zc[j] = cen + rh * tc[j];
-        wc[j] = std::complex<double>(rv * std::cos(th), rh * std::sin(th)) / (double)N;
-    }
-    if (p.conjugate_pairs)
-        for (int j = N / 2; j < N; ++j) {
-            zc[j] = std::conj(zc[N - 1 - j]);
-            tc[j] = std::conj(tc[N - 1 - j]);
-            wc[j] = std::conj(wc[N - 1 - j]);
+    // contour per k point (ciss.py:81-104), FP64 then complex64; every k has
+    // the request's interval unless per-k intervals are given (serving batches
+    // k points of different jobs into one factorisation launch)
+    struct Contour {
+        std::vector<std::complex<double>> z;
+        std::vector<std::complex<float>> t32, w32;
+        double lower, upper;
+    };
+    auto make_contour = [&](double lower, double upper) {
+        Contour c;
+        c.lower = lower;
+        c.upper = upper;
+        const double lo = lower - 2.0 * p.guard, hi = upper + 2.0 * p.guard;
+        const double cen = 0.5 * (lo + hi), rh = 0.5 * (hi - lo), rv = rh * p.vscale;
+        std::vector<std::complex<double>> tc(N), wc(N);
+        c.z.resize(N);
+        for (int j = 0; j < N; ++j) {
+            const double th = 2.0 * M_PI * (j + 0.5) / N;
+            tc[j] = {std::cos(th), p.vscale * std::sin(th)};
+            c.z[j] = cen + rh * tc[j];
+            wc[j] = std::complex<double>(rv * std::cos(th), rh * std::sin(th)) / (double)N;
         }
+        if (p.conjugate_pairs)
+            for (int j = N / 2; j < N; ++j) {
+                c.z[j] = std::conj(c.z[N - 1 - j]);
+                tc[j] = std::conj(tc[N - 1 - j]);
+                wc[j] = std::conj(wc[N - 1 - j]);
+            }
+        c.t32.resize(N);
+        c.w32.resize(N);
+        for (int j = 0; j < N; ++j) {
+            c.t32[j] = {(float)tc[j].real(), (float)tc[j].imag()};
+            c.w32[j] = {(float)wc[j].real(), (float)wc[j].imag()};
+        }
+        return c;
+    };
+    std::vector<Contour> contours(p.nk);
+    for (int64_t gk = 0; gk < p.nk; ++gk)
+        contours[gk] = p.intervals ? make_contour(p.intervals[2 * gk], p.intervals[2 * gk + 1])
+                                   : make_contour(p.lower, p.upper);
     auto f2 = [](std::complex<double> v) { return make_float2((float)v.real(), (float)v.imag()); };
-    std::vector<std::complex<float>> t32(N), w32(N);
-    for (int j = 0; j < N; ++j) {
-        t32[j] = {(float)tc[j].real(), (float)tc[j].imag()};
-        w32[j] = {(float)wc[j].real(), (float)wc[j].imag()};
-    }
-    // moment coefficient of shift j, moment m: -w_j t_j^m in complex64 arithmetic
-    auto coef = [&](int j, int m) {
+    // moment coefficient of shift j, moment m for k point gk: -w_j t_j^m in
+    // complex64 arithmetic
+    auto coef = [&](int64_t gk, int j, int m) {
+        const Contour &c = contours[gk];
         std::complex<float> tp(1.f, 0.f);
-        for (int q = 0; q < m; ++q) tp *= t32[j];
-        const std::complex<float> c = -w32[j] * tp;
-        return make_float2(c.real(), c.imag());
+        for (int q = 0; q < m; ++q) tp *= c.t32[j];
+        const std::complex<float> v = -c.w32[j] * tp;
+        return make_float2(v.real(), v.imag());
     };
     float2 *d_V = buf<float2>("probes", (size_t)n * L);
     AOSX_HIP(hipMemcpyAsync(d_V, probes, (size_t)n * L * sizeof(float2), hipMemcpyHostToDevice, stream));
@@ -704,8 +725,8 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 for (int j = j0; j < j1; ++j) {
                     gr.kidx.push_back(k);
                     gr.shift.push_back(j);
-                    gr.z.push_back(f2(zc[j]));
-                    gr.zc.push_back(f2(std::conj(zc[j])));
+                    gr.z.push_back(f2(contours[kb + k].z[j]));
+                    gr.zc.push_back(f2(std::conj(contours[kb + k].z[j])));
                 }
             gr.slot0 = resident ? g * G : 0;
         }
@@ -717,12 +738,8 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             std::vector<float2> cf(gr.kidx.size() * msize);
             for (size_t b = 0; b < gr.kidx.size(); ++b)
                 for (int m = 0; m < msize; ++m) {
-                    float2 c = coef(gr.shift[b], m);
-                    if (conj_side) {
-                        const int jj = N - 1 - gr.shift[b];
-                        c = coef(jj, m);
-                    }
-                    cf[b * msize + m] = c;
+                    const int64_t gk = kb + gr.kidx[b];
+                    cf[b * msize + m] = conj_side ? coef(gk, N - 1 - gr.shift[b], m) : coef(gk, gr.shift[b], m);
                 }
             AOSX_HIP(hipMemcpyAsync(d_coef, cf.data(), cf.size() * sizeof(float2), hipMemcpyHostToDevice, stream));
             for (int k = 0; k < Kc; ++k) {
@@ -763,7 +780,12 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
         // subspace stage
         Subspace sub;
         const char *ritz_names[2] = {"b_ritz0", "b_ritz1"};
-        compress_and_project(*this, Kc, d_M, P, d_H, d_S, p, sub, out.stages,
+        std::vector<double> klo(Kc), khi(Kc);
+        for (int k = 0; k < Kc; ++k) {
+            klo[k] = contours[kb + k].lower;
+            khi[k] = contours[kb + k].upper;
+        }
+        compress_and_project(*this, Kc, d_M, P, d_H, d_S, p, klo.data(), khi.data(), sub, out.stages,
                              p.refinement_passes > 0 ? ritz_names[0] : nullptr);
         std::vector<int> rank0 = sub.rank;
         // filter passes: the block [Ritz vectors | augmentation] is filtered
@@ -825,7 +847,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             out.stages[9] += now() - tp;
             const bool more = pass + 1 < p.refinement_passes;
             Subspace next;
-            compress_and_project(*this, Kc, d_Y, W2, d_H, d_S, p, next, out.stages,
+            compress_and_project(*this, Kc, d_Y, W2, d_H, d_S, p, klo.data(), khi.data(), next, out.stages,
                                  more ? ritz_names[(pass + 1) & 1] : nullptr);
             if (std::getenv("AOSX_DEBUG"))
                 for (int k = 0; k < Kc; ++k)

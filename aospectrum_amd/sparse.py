@@ -20,7 +20,7 @@ from scipy import sparse
 
 from . import _native as N
 from .data import LocalizedOperatorBundle
-from .errors import InputError
+from .errors import InputError, SolverError
 
 
 @dataclass(slots=True)
@@ -141,6 +141,13 @@ class NativeBundle:
         res = N.StatusResult()
         N.check(N.load().aosx_assemble_v1(self.handle, C.byref(req), C.byref(res)), res, "Bloch assembly")
         return h, s
+
+    def trim(self) -> int:
+        """Free the device workspace; returns the bytes still resident."""
+        held = C.c_uint64(0)
+        if N.load().aosx_bundle_trim_v1(self.handle, C.byref(held)) != 0:
+            raise SolverError("cannot release the bundle workspace")
+        return int(held.value)
 
     def kernel_stats(self, enable: bool) -> dict:
         out = N.KernelStats()

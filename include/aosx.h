@@ -160,6 +160,13 @@ int aosx_bundle_pattern_v1(const aosx_bundle *bundle, int32_t *indptr, int32_t *
 
 int aosx_bundle_destroy_v1(aosx_bundle *bundle);
 
+/* Frees the bundle's device workspace (factors, right-hand sides, subspace
+ * buffers) while keeping its resident block data and symbolic analysis.  No
+ * counterpart in the reference, which frees everything after every call; the
+ * serving loop uses it to hand memory from an idle bundle to another job.
+ * Writes the bytes still held in *resident_bytes (may be NULL). */
+int aosx_bundle_trim_v1(aosx_bundle *bundle, uint64_t *resident_bytes);
+
 /* ---------------------------------------------------------------- assemble */
 typedef struct {
     uint32_t abi_version;
@@ -219,6 +226,7 @@ typedef struct {
     const aosx_complex64 *s_data;
     double lower_ev;               /* absolute solve interval                      */
     double upper_ev;
+    const double *intervals;       /* [n_kpoints][2] per-k intervals, or NULL      */
     /* CissSettings (src/aospectrum/ciss.py:35-62) */
     int32_t integration_points;
     int32_t contour_batch_size;
