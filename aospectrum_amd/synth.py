@@ -17,8 +17,14 @@ One calibration knob is added: onsite diagonal energies are drawn from
 ``U[-12, 4]`` with the band ``(gap_center - gap, gap_center + gap)`` removed.
 With the literal recipe the CISS windows would hold hundreds to thousands of
 states, far beyond a 128-column search space; the gap half-widths below were
-chosen by measuring the in-window count with an FP64 dense eigensolver.  Every
-spec records its seed and knob in the bundle provenance.
+chosen by measuring the in-window count with an FP64 dense eigensolver or
+Sylvester inertia: small2d 40-41 states in [-2.02, 2.02] eV, stress2d 64 in
+[-1.52, 1.52] eV, bulk3d 66-67 in [-1.02, 1.02] eV.  Every spec records its
+seed and knob in the bundle provenance.
+
+The precision-stress bundle (stress2d) clusters its onsite levels on 96
+centres with 1e-3 eV spread and scales the off-site overlap so that the
+smallest eigenvalue of S(k) is 1.0e-3 along the path (no row-sum rescale).
 """
 
 from __future__ import annotations
@@ -85,9 +91,11 @@ SPECS: dict[str, SynthSpec] = {
         (40.0, 40.0, 30.0),
         (True, True, False),
         1001,
-        gap=6.0,
-        s_amplitude=0.25,
-        s_row_bound=0.999,
+        gap=2.2,
+        # no row-sum rescale; the amplitude puts min eig S(k) at 1.0e-3 on the
+        # Y-G-X-M path (measured: unscaled off-site overlap has min eig -3.2555)
+        s_amplitude=0.30686,
+        s_row_bound=1.0e30,
         cluster_spread=1.0e-3,
         cluster_count=96,
     ),

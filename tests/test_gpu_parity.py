@@ -202,3 +202,44 @@ def test_bulk3d_matches_golden_inertia_counts():
         say(f"  count={r.energies_ev.size} golden={count} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e}")
         assert r.energies_ev.size == count
         assert r.maximum_residual_ev <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def stress2d():
+    return synth.generate(synth.SPECS["stress2d"])
+
+
+@pytest.mark.parametrize(
+    "label,settings,refined",
+    [
+        # complex64 FP32 LU without refinement, reference launch shape, with
+        # the filter passes (one pass alone misses states bunched at a window
+        # end, as the reference algorithm does)
+        ("plain_c64", CissSettings(refinement_passes=2), False),
+        ("fp32_fp64ir", CissSettings(precision="fp32_ir", refinement_steps=2, refinement_passes=2), True),
+        ("bf16x3_fp64ir", CissSettings(contour_batch_size=16, k_batch=2, conjugate_pairs=True,
+                                       precision="bf16x3_ir", refinement_steps=2, refinement_passes=2), True),
+    ],
+)
+def test_precision_stress_against_fp64(stress2d, label, settings, refined):
+    """3,328 orbitals, min eig S(k) = 1e-3, clustered onsite levels: every
+    precision mode against the FP64 dense generalized eigensolver."""
+    pat = bloch.Pattern.build(stress2d.operators, stress2d.basis.atom_offsets, stress2d.n_orbitals)
+    kpts = np.array([[0.0, 0.0, 0.0], [0.5, 0.25, 0.0]])
+    interval = EnergyInterval(-1.52, 1.52)
+    say("stress2d", label)
+    nb = NativeBundle(stress2d, device=0)
+    try:
+        results, info = band_solve(nb, settings, interval, kpoints=kpts)
+    finally:
+        nb.close()
+    for k, r in zip(kpts, results):
+        h, s = pat.assemble(stress2d.operators, k)
+        ref = dense.window_spectrum(pat.csr(h).toarray(), pat.csr(s).toarray(), interval.lower_ev, interval.upper_ev)
+        err = float(np.max(np.abs(np.sort(r.energies_ev) - ref))) if r.energies_ev.size == ref.size else np.inf
+        say(f"  k={k} count={r.energies_ev.size} ref={ref.size} rank={r.subspace_rank} "
+            f"max|dE|={err:.2e} res={r.maximum_residual_ev:.2e} pivots={info['replaced_pivots']}")
+        assert r.energies_ev.size == ref.size
+        assert err < 1e-4
+        if refined:
+            assert r.maximum_residual_ev <= 1e-4
