@@ -127,10 +127,12 @@ class BandServer:
         device: int = 0,
         memory_budget_bytes: int = 200 * 2**30,
         k_batch: int = 4,
+        on_batch=None,
     ) -> None:
         self.device = device
         self.budget = int(memory_budget_bytes)
         self.k_batch = int(k_batch)
+        self.on_batch = on_batch
         self._lock = threading.Condition()
         self._bundles: dict[str, _Resident] = {}
         self._closed = False
@@ -228,6 +230,10 @@ class BandServer:
                 if not batch:
                     continue
                 settings = batch[0].job.job.settings
+                # a batch that could never fit the budget is cut down (one k
+                # point always fits: submit() rejects jobs where it does not)
+                while len(batch) > 1 and self.estimate_bytes(res.key, settings, len(batch)) > self.budget:
+                    res.pending.appendleft(batch.pop())
                 need = self.estimate_bytes(res.key, settings, len(batch))
                 while not self._reserve(res, need):
                     self._lock.wait(timeout=0.05)
@@ -250,8 +256,11 @@ class BandServer:
         except Exception as exc:  # noqa: BLE001 -- reported per job
             results, info, failure = None, {}, f"{type(exc).__name__}: {exc}"
         ended = time.perf_counter()
-        self.batch_log.append({"bundle": res.key, "k_points": len(batch), "jobs": len(jobs),
-                               "start": started, "end": ended})
+        entry = {"bundle": res.key, "k_points": len(batch), "jobs": len(jobs), "start": started, "end": ended,
+                 "error": failure}
+        self.batch_log.append(entry)
+        if self.on_batch is not None:
+            self.on_batch(entry)
         with self._lock:
             for pos, it in enumerate(batch):
                 st = it.job

@@ -71,6 +71,113 @@ def cpu_baseline(bundle, kpoint, interval, shifts_sampled: int) -> dict:
     }
 
 
+def serving_bench(args) -> int:
+    """Mixed request load through the continuous-batching server.
+
+    Jobs arrive as a Poisson stream (mean ``--rate`` jobs/s, seed 4242).  Each
+    draws a bundle uniformly from the serving set, a k-path length from
+    {20, 61, 121} (scaled by ``--kscale`` to keep the run bounded, points spread
+    evenly over the Y-G-X-M path), a window half-width U[0.5, 2] eV about 0 and
+    a block size from {32, 64}.  Reported: p50/p99 job latency, aggregate
+    k-points/s, and the same jobs as a sequential plain-CISS loop on the GPU.
+    """
+    from aospectrum_amd import synth
+    from aospectrum_amd.band import ygxm_path
+    from aospectrum_amd.ciss import CissSettings, band_solve
+    from aospectrum_amd.data import EnergyInterval
+    from aospectrum_amd.serving import BandJob, BandServer, latency_summary
+
+    names = args.serving_bundles.split(",")
+    bundles = {name: synth.generate(synth.SPECS[name]) for name in names}
+    rng = np.random.default_rng(4242)
+    path = ygxm_path(40)
+    jobs = []
+    t = 0.0
+    for j in range(args.jobs):
+        t += rng.exponential(1.0 / args.rate)
+        name = names[rng.integers(len(names))]
+        length = max(1, int(round(int(rng.choice([20, 61, 121])) * args.kscale)))
+        idx = np.unique(np.linspace(0, path.n_points - 1, length).round().astype(int))
+        half = float(rng.uniform(0.5, 2.0))
+        block = int(rng.choice([32, 64]))
+        st = CissSettings(block_size=block, contour_batch_size=16, conjugate_pairs=True,
+                          precision=args.precision, refinement_steps=args.ir_steps, refinement_passes=args.passes)
+        jobs.append((t, BandJob(name, path.fractional_points[idx], EnergyInterval(-half - 0.02, half + 0.02), st,
+                                f"job{j}-{name}-{idx.size}k-h{half:.2f}-L{block}")))
+    log(f"serving: {len(jobs)} jobs over {[n for n in names]}, {sum(j.kpoints.shape[0] for _, j in jobs)} k points")
+    results, failures = [], []
+
+    def progress(entry):
+        log(f"batch {entry['bundle']}: {entry['k_points']} k from {entry['jobs']} job(s), "
+            f"{entry['end'] - entry['start']:.2f} s" + (f", error {entry['error'][:80]}" if entry["error"] else ""))
+
+    with BandServer(k_batch=args.k_batch, on_batch=progress) as server:
+        for name, b in bundles.items():
+            server.register(name, b)
+        t0 = time.perf_counter()
+        futures = []
+        for arrival, job in jobs:
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futures.append((job, server.submit(job)))
+        for job, fut in futures:
+            try:
+                results.append(fut.result())
+                log(f"done {job.job_id}: latency {results[-1].latency_s:.2f} s")
+            except Exception as exc:  # noqa: BLE001 -- failed jobs are reported
+                failures.append({"job": job.job_id, "error": str(exc)[:200]})
+                log(f"failed {job.job_id}: {str(exc)[:120]}")
+        batches = list(server.batch_log)
+    summary = latency_summary(results) if results else {}
+    log(f"serving done: {summary}, failed {len(failures)}")
+    # the same successful jobs, one after another, plain configuration
+    seq_rate = None
+    if not args.no_plain and results:
+        from aospectrum_amd.sparse import NativeBundle
+
+        natives = {name: NativeBundle(b, device=0) for name, b in bundles.items()}
+        done = {r.job_id for r in results}
+        t1 = time.perf_counter()
+        nk = 0
+        for _, job in jobs:
+            if job.job_id not in done:
+                continue
+            plain = CissSettings(block_size=job.settings.block_size, precision="fp32_ir", refinement_steps=2,
+                                 refinement_passes=args.passes)
+            band_solve(natives[job.bundle_key], plain, job.interval, kpoints=job.kpoints)
+            nk += job.kpoints.shape[0]
+            log(f"plain {job.job_id} done")
+        seq_rate = nk / (time.perf_counter() - t1)
+        for nb in natives.values():
+            nb.close()
+        log(f"sequential plain loop: {seq_rate:.3f} k-points/s")
+    line = {
+        "metric": "serving: p50/p99 job latency and aggregate k-points/s under mixed request load",
+        "value": summary.get("kpoints_per_s"),
+        "unit": "k-points/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "p50_latency_s": summary.get("p50_latency_s"),
+        "p99_latency_s": summary.get("p99_latency_s"),
+        "jobs_completed": len(results),
+        "jobs_failed": failures,
+        "sequential_plain_kpoints_per_s": seq_rate,
+        "speedup_vs_sequential_plain": (summary["kpoints_per_s"] / seq_rate) if seq_rate and summary else None,
+        "batches": len(batches),
+        "batches_shared_by_jobs": sum(1 for b in batches if b["jobs"] > 1),
+        "config": {
+            "workload": f"Poisson {args.rate} jobs/s, {args.jobs} jobs, bundles {names}, k-path lengths "
+            f"{{20,61,121}} x {args.kscale}, half-width U[0.5,2] eV, block {{32,64}}",
+            "precision": args.precision,
+            "k_batch": args.k_batch,
+        },
+        "data": "synthetic (seeded generator)",
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +192,11 @@ def main() -> int:
     ap.add_argument("--cpu-shifts", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serving", action="store_true", help="mixed request load through the serving loop")
+    ap.add_argument("--serving-bundles", default="small2d,stress2d,bulk3d")
+    ap.add_argument("--jobs", type=int, default=12)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--kscale", type=float, default=0.1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -93,6 +205,8 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if args.serving:
+        return serving_bench(args)
 
     from aospectrum_amd import synth
     from aospectrum_amd.band import ygxm_path
