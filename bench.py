@@ -38,6 +38,14 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0}
 
+# HBM bytes per k_schur dispatch (workload, precision) -> bytes, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this bench, scaled by what
+# the counters read for a copy kernel of known size; the commands and the
+# per-kernel table are in profiles/k_schur_bulk3d_traffic.md.  Counters
+# cannot be read from inside a timed run, so the figure is carried here and
+# refreshed whenever the kernel changes.
+MEASURED_SCHUR_TRAFFIC = {("bulk3d", "bf16x3_ir"): 57.27e9}
+
 
 def log(*parts):
     print("[bench]", *parts, file=sys.stderr, flush=True)
@@ -189,7 +197,7 @@ def main() -> int:
     ap.add_argument("--precision", default="bf16x3_ir")
     ap.add_argument("--ir-steps", type=int, default=2)
     ap.add_argument("--passes", type=int, default=2)
-    ap.add_argument("--cpu-shifts", type=int, default=2)
+    ap.add_argument("--cpu-shifts", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serving", action="store_true", help="mixed request load through the serving loop")
@@ -363,7 +371,8 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": MEASURED_SCHUR_TRAFFIC.get((args.workload, args.precision)),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-scaled)",
             "launches": ks["schur_launches"],
             "avg_launch_ms": ks["schur_ms"] / max(ks["schur_launches"], 1),
         },
