@@ -226,6 +226,9 @@ class KernelStats(C.Structure):
         ("assemble_launches", C.c_int64),
         ("assemble_ms", C.c_double),
         ("assemble_bytes", C.c_double),
+        ("inner_launches", C.c_int64),
+        ("inner_flops", C.c_double),
+        ("inner_ms", C.c_double),
     ]
 
 

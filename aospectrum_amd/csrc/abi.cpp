@@ -404,6 +404,9 @@ int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_s
     out->assemble_launches = k.assemble_launches;
     out->assemble_ms = k.assemble_ms;
     out->assemble_bytes = k.assemble_bytes;
+    out->inner_launches = k.inner_launches;
+    out->inner_flops = k.inner_flops;
+    out->inner_ms = k.inner_ms;
     b.kstats = aosx::KernelStats();
     b.kstats.enabled = enable != 0;
     return 0;

@@ -32,9 +32,12 @@ struct Error : std::runtime_error {
 
 struct KernelStats {
     bool enabled = false;
-    int64_t schur_launches = 0;
+    int64_t schur_launches = 0;     // trailing (grouped) Schur updates
     double schur_flops = 0.0;
     double schur_ms = 0.0;
+    int64_t inner_launches = 0;     // Schur updates inside a step group
+    double inner_flops = 0.0;
+    double inner_ms = 0.0;
     int64_t panel_launches = 0;
     double panel_ms = 0.0;
     int64_t getrf_launches = 0;
@@ -80,6 +83,24 @@ struct Bundle {
     int32_t *d_rows = nullptr, *d_indices = nullptr, *d_indptr = nullptr, *d_shifts = nullptr;
     int32_t *d_iperm = nullptr, *d_perm = nullptr, *d_tile_id = nullptr, *d_next = nullptr, *d_prev = nullptr;
     int32_t *d_atom_off = nullptr;   // [n_atoms + 1] orbital offsets (atom-block SpMM)
+    // Step groups of the factorisation: steps [grp_begin[g], grp_begin[g+1])
+    // update the tiles of their own rows and columns step by step; all other
+    // targets take the group's updates in one trailing launch over the pair
+    // list d_pairs[grp_pair_ptr[g] .. grp_pair_ptr[g+1]) (tile indices i, j).
+    int schur_group = 16;
+    std::vector<int32_t> grp_begin;
+    std::vector<int64_t> grp_pair_ptr;
+    std::vector<double> grp_updates;  // tile updates of each trailing launch
+    int32_t *d_pairs = nullptr;
+    // BF16 precisions: the panels of a group also write split copies of their
+    // tiles; L(i,t) sits in slot ppos[t*nt+i] of the group's split area and
+    // U(t,j) in slot ppos[t*nt+j] + step_s[t] (split_tiles slots per matrix).
+    int32_t *d_ppos = nullptr, *d_step_s = nullptr;
+    int64_t split_tiles = 0;
+    // Solve sweeps use the same step groups: rows outside group g that its
+    // steps couple to, forward (i >= g1) and backward (i < g0), ascending.
+    std::vector<int64_t> fwd_row_ptr, bwd_row_ptr;
+    int32_t *d_fwd_rows = nullptr, *d_bwd_rows = nullptr;
     int64_t n_atoms = 0;
     int max_atom = 0;                // largest atom, in orbitals
     uint64_t resident_bytes = 0;
@@ -97,6 +118,7 @@ struct Bundle {
     T *buf(const std::string &name, size_t count) {
         return static_cast<T *>(raw(name, count * sizeof(T)));
     }
+    void plan_schur(int group);
     void release_workspace();
     uint64_t workspace_bytes() const;
     size_t factor_floats() const { return (size_t)ts.n_tiles * 2 * 64 * 64; }

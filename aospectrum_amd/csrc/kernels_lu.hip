@@ -6,17 +6,24 @@
 // tile structure comes from the host symbolic analysis; for step k the tiles
 // next[k] hold L(i,k) and, by symmetry of the pattern, U(k,i).
 //
-// Per step k (right-looking):
+// Steps are taken in groups of consecutive steps [g0, g1) (right-looking
+// across groups).  Per step k of a group:
 //   getrf  : factor the diagonal tile without row exchanges; a pivot below
 //            1e-6 * max|tile| is replaced by that bound (static pivoting in the
 //            sense of Li & Demmel's GESP, corrected later by refinement);
 //            also writes inv(L_kk) (unit lower) and inv(U_kk).
 //   panel  : L(i,k) <- L(i,k) * inv(U_kk),  U(k,i) <- inv(L_kk) * U(k,i)   (MFMA)
-//   schur  : A(i,j) -= L(i,k) * U(k,j) for i, j in next[k]                  (MFMA)
+//   schur  : A(i,j) -= L(i,k) * U(k,j) for i, j in next[k] with i < g1 or
+//            j < g1 (the tiles the group's later steps need)              (MFMA)
+// and after the group's last step one trailing launch applies
+//   A(i,j) -= sum_{k in group} L(i,k) * U(k,j)   for i, j >= g1,
+// keeping A(i,j) in the accumulators, so each trailing target is read and
+// written once per group.
 // The complex products use four real FP32 MFMA products (v_mfma_f32_32x32x2_f32),
-// i.e. 8 real flops per complex multiply-add.  The BF16 variants split every
-// FP32 operand into hi + lo BF16 parts (v_mfma_f32_32x32x16_bf16) and keep FP32
-// accumulation.
+// i.e. 8 real flops per complex multiply-add.  In the BF16 precisions the
+// panels also write their tiles split into hi + lo BF16 parts, which the Schur
+// updates feed to v_mfma_f32_32x32x16_bf16 (three products, FP32
+// accumulation); the stored factor stays FP32.
 #include "device_common.h"
 
 namespace aosx {
@@ -297,12 +304,65 @@ __device__ __forceinline__ void store_acc(float *__restrict__ t, const floatx16 
     }
 }
 
+// Split copies of panel tiles for the BF16 trailing update.  A split tile is
+// four planes of 64 x 64 BF16 -- re_hi, re_lo, im_hi, im_lo, with
+// x ~ hi + lo, hi = bf16(x), lo = bf16(x - hi) (round to nearest even) --
+// holding -L(i,k) row-major ([row][col]) or U(k,j) transposed ([col][row]), so
+// that both MFMA operands read 8 consecutive k values per lane.
+constexpr int SPLIT_TILE = 4 * TILE2;  // BF16 elements per split tile (32 KiB)
+
+__device__ __forceinline__ void split_pair(float x, uint16_t &hi, uint16_t &lo) {
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, l);
+}
+
+// Writes the tile held in LDS (re/im planes, row stride LDA) as a split tile.
+// negate_rows: -T row-major (the L operand); otherwise T transposed (the U
+// operand).  Thread tid covers 16 consecutive destination elements of one
+// destination row, written as 2 x 16 B per plane.
+__device__ __forceinline__ void write_split(const float *Tre, const float *Tim, uint16_t *__restrict__ dst,
+                                            bool negate_rows, int tid) {
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    uint16_t rh[16], rl[16], ih[16], il[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        float xr, xi;
+        if (negate_rows) {
+            xr = -Tre[r * LDA + c0 + c];
+            xi = -Tim[r * LDA + c0 + c];
+        } else {
+            xr = Tre[(c0 + c) * LDA + r];
+            xi = Tim[(c0 + c) * LDA + r];
+        }
+        split_pair(xr, rh[c], rl[c]);
+        split_pair(xi, ih[c], il[c]);
+    }
+    const int off = r * TILE + c0;
+    uint4 *d0 = reinterpret_cast<uint4 *>(dst + off);
+    uint4 *d1 = reinterpret_cast<uint4 *>(dst + TILE2 + off);
+    uint4 *d2 = reinterpret_cast<uint4 *>(dst + 2 * TILE2 + off);
+    uint4 *d3 = reinterpret_cast<uint4 *>(dst + 3 * TILE2 + off);
+    auto pack = [](const uint16_t *v, int o) {
+        return make_uint4(v[o] | ((uint32_t)v[o + 1] << 16), v[o + 2] | ((uint32_t)v[o + 3] << 16),
+                          v[o + 4] | ((uint32_t)v[o + 5] << 16), v[o + 6] | ((uint32_t)v[o + 7] << 16));
+    };
+    d0[0] = pack(rh, 0); d0[1] = pack(rh, 8);
+    d1[0] = pack(rl, 0); d1[1] = pack(rl, 8);
+    d2[0] = pack(ih, 0); d2[1] = pack(ih, 8);
+    d3[0] = pack(il, 0); d3[1] = pack(il, 8);
+}
+
 // Panel step k: block x < s updates L(next[x], k) <- L * inv(U_kk); block
-// x >= s updates U(k, next[x - s]) <- inv(L_kk) * U.
+// x >= s updates U(k, next[x - s]) <- inv(L_kk) * U.  With `split` set, the
+// result is also written as split tile slot0 + x of the matrix's split area
+// (split_stride split tiles per matrix).
 __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_t mat_stride,
                                               const float *__restrict__ dinv, int nt, int k,
                                               const int32_t *__restrict__ next, int s,
-                                              const int32_t *__restrict__ tile_id) {
+                                              const int32_t *__restrict__ tile_id, uint16_t *__restrict__ split,
+                                              int64_t split_stride, int slot0) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -322,18 +382,43 @@ __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_
     floatx16 cre = {}, cim = {};
     mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
     store_acc(t, cre, cim, wave, lane);
+    if (split != nullptr) {
+        __syncthreads();  // every wave is done reading the operands
+        const int r0 = (wave >> 1) * 32 + 4 * (lane >> 5), c = (wave & 1) * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = r0 + 8 * (r >> 2) + (r & 3);
+            Are[row * LDA + c] = cre[r];
+            Aim[row * LDA + c] = cim[r];
+        }
+        __syncthreads();
+        uint16_t *dst = split + ((int64_t)m * split_stride + slot0 + blockIdx.x) * SPLIT_TILE;
+        write_split(Are, Aim, dst, lower, tid);
+    }
 }
 
-// Schur step k: A(i,j) -= L(i,k) U(k,j) for the s*s pairs of next[k].
-// MODE 0: FP32 MFMA; 1: BF16 x3 split; 2: BF16 x1.
-template <int MODE>
+// Schur step k inside a step group: A(i,j) -= L(i,k) U(k,j) for the pairs
+// (next[a], next[b]) with a < p or b < p, where next[0..p) are the tiles of
+// the group (the rest take this step's update in the group's trailing launch).
+// Block x < p*s is pair (x / s, x % s); the (s-p)*p blocks after it are
+// (p + y / p, y % p).  With p == s this is the whole s*s Schur complement.
+// FP32 MFMA (the BF16 precisions use k_schur_split).
 __global__ void __launch_bounds__(256) k_schur(float *__restrict__ tiles, int64_t mat_stride, int nt, int k,
-                                              const int32_t *__restrict__ next, int s,
+                                              const int32_t *__restrict__ next, int s, int p,
                                               const int32_t *__restrict__ tile_id) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int a = blockIdx.x / s, b = blockIdx.x % s;
+    const int x = blockIdx.x;
+    int a, b;
+    if (x < p * s) {
+        a = x / s;
+        b = x % s;
+    } else {
+        const int y = x - p * s;
+        a = p + y / p;
+        b = y % p;
+    }
     const int i = next[a], j = next[b];
     const float *L = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + k]);
     const float *U = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)k * nt + j]);
@@ -343,13 +428,211 @@ __global__ void __launch_bounds__(256) k_schur(float *__restrict__ tiles, int64_
     floatx16 cre, cim;
     load_acc(C, cre, cim, wave, lane);
     __syncthreads();
-    if (MODE == 0) mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
-    else mma_bf16_sub<MODE == 1>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+    mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
     store_acc(C, cre, cim, wave, lane);
 }
 
-template __global__ void k_schur<0>(float *, int64_t, int, int, const int32_t *, int, const int32_t *);
-template __global__ void k_schur<1>(float *, int64_t, int, int, const int32_t *, int, const int32_t *);
-template __global__ void k_schur<2>(float *, int64_t, int, int, const int32_t *, int, const int32_t *);
+// Trailing update of a step group [g0, g1): A(i,j) -= sum_t L(i,t) U(t,j) over
+// the steps t of the group where both tiles exist, for every pair (i, j) of
+// the group's list (i, j >= g1).  The target stays in the MFMA accumulators
+// across the group's steps, so it is read and written once per group rather
+// than once per step.  The grid is 8 * ceil(npairs * nmat / 8) blocks in one
+// dimension; consecutive blocks go round-robin to the 8 XCDs, so block b works
+// on item (b % 8) * per + b / 8 and each XCD (with its own L2) takes one
+// contiguous run of the (matrix, pair) items, whose pairs share L rows and U
+// columns.  FP32 MFMA (the BF16 precisions use k_schur_group_split).
+__global__ void __launch_bounds__(256) k_schur_group(float *__restrict__ tiles, int64_t mat_stride, int nt, int g0,
+                                                    int g1, const int32_t *__restrict__ pairs, int npairs, int nmat,
+                                                    const int32_t *__restrict__ tile_id) {
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t total = (int64_t)npairs * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = (int)(item / npairs), q = (int)(item % npairs);
+    const int i = pairs[2 * q], j = pairs[2 * q + 1];
+    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
+    floatx16 cre, cim;
+    load_acc(C, cre, cim, wave, lane);
+    for (int t = g0; t < g1; ++t) {
+        const int32_t lid = tile_id[(int64_t)i * nt + t], uid = tile_id[(int64_t)t * nt + j];
+        if (lid < 0 || uid < 0) continue;
+        __syncthreads();  // the previous step's products are done with LDS
+        stage_a(tile_ptr(tiles, mat_stride, m, lid), Are, Aim, tid);
+        stage_b(tile_ptr(tiles, mat_stride, m, uid), Bre, Bim, tid);
+        __syncthreads();
+        mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+    }
+    store_acc(C, cre, cim, wave, lane);
+}
+
+// Trailing update of a step group from split operands (BF16 precisions):
+// the same pairs and step loop as k_schur_group, with L and U read from the
+// split tiles the panels of the group wrote.  Slot of L(i,t) in the matrix's
+// split area: ppos[t * nt + i] (-1 when absent); slot of U(t,j):
+// ppos[t * nt + j] + step_s[t].  The next step's two split tiles are loaded
+// into registers while the MFMA products of the current step run.
+constexpr int SLD = TILE + 8;          // LDS row stride (BF16) of a split plane: 144 B
+constexpr int SPL = TILE * SLD;        // LDS elements per split plane
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, v) ^ (u32x4)(0x80008000u));
+}
+
+// acc += P * U with P = -L, both split: re += Pr Ur - Pi Ui, im += Pr Ui + Pi Ur,
+// each real product as hi*hi + hi*lo + lo*hi (THREE) or hi*hi.
+template <bool THREE>
+__device__ __forceinline__ void mma_split(const uint16_t *As, const uint16_t *Bs, floatx16 &cre, floatx16 &cim,
+                                          int wave, int lane) {
+    const int i = (wave >> 1) * 32 + (lane & 31);
+    const int j = (wave & 1) * 32 + (lane & 31);
+    const int kb = (lane >> 5) * 8;
+#pragma unroll
+    for (int k0 = 0; k0 < TILE; k0 += 16) {
+        const int ao = i * SLD + k0 + kb, bo = j * SLD + k0 + kb;
+        const bf16x8 prh = lds8(As + ao), pih = lds8(As + 2 * SPL + ao);
+        const bf16x8 urh = lds8(Bs + bo), uih = lds8(Bs + 2 * SPL + bo);
+        const bf16x8 npih = neg8(pih);
+        cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, urh, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uih, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, uih, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, urh, cim, 0, 0, 0);
+        if (THREE) {
+            const bf16x8 prl = lds8(As + SPL + ao), pil = lds8(As + 3 * SPL + ao);
+            const bf16x8 url = lds8(Bs + SPL + bo), uil = lds8(Bs + 3 * SPL + bo);
+            const bf16x8 npil = neg8(pil);
+            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, url, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl, urh, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uil, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npil, uih, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, uil, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl, uih, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, url, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pil, urh, cim, 0, 0, 0);
+        }
+    }
+}
+
+// One split tile = 2048 chunks of 16 B; thread tid moves chunks q * 256 + tid.
+// The chunks travel in a register vector (not an array, which the compiler
+// may place in scratch memory when it is carried across loop iterations).
+typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
+
+__device__ __forceinline__ u32x32 fetch_split(const uint16_t *__restrict__ src, int tid) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    u32x32 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint4 v = s4[q * 256 + tid];
+        r[4 * q] = v.x;
+        r[4 * q + 1] = v.y;
+        r[4 * q + 2] = v.z;
+        r[4 * q + 3] = v.w;
+    }
+    return r;
+}
+__device__ __forceinline__ void put_split(uint16_t *dst, const u32x32 r, int tid) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = q * 256 + tid;                 // chunk in the tile
+        const int plane = e >> 9, row = (e >> 3) & 63, c8 = e & 7;
+        *reinterpret_cast<uint4 *>(dst + plane * SPL + row * SLD + c8 * 8) =
+            make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    }
+}
+
+template <bool THREE>
+__global__ void __launch_bounds__(256, 2) k_schur_group_split(
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ pairs,
+    int npairs, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
+    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t total = (int64_t)npairs * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = (int)(item / npairs), q = (int)(item % npairs);
+    const int i = pairs[2 * q], j = pairs[2 * q + 1];
+    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
+    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
+    // steps of the group with both L(i,t) and U(t,j) present (groups <= 32 steps)
+    uint32_t mask = 0;
+    for (int t = g0; t < g1; ++t)
+        if (ppos[(int64_t)t * nt + i] >= 0 && ppos[(int64_t)t * nt + j] >= 0) mask |= 1u << (t - g0);
+    floatx16 cre, cim;
+    load_acc(C, cre, cim, wave, lane);
+    if (mask) {
+        int t = g0 + __builtin_ctz(mask);
+        mask &= mask - 1;
+        u32x32 ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
+        u32x32 rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
+        for (;;) {
+            __syncthreads();  // the previous step's products are done with LDS
+            put_split(As, ra, tid);
+            put_split(Bs, rb, tid);
+            __syncthreads();
+            const bool more = mask != 0;
+            if (more) {
+                t = g0 + __builtin_ctz(mask);
+                mask &= mask - 1;
+                ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
+                rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
+            }
+            mma_split<THREE>(As, Bs, cre, cim, wave, lane);
+            if (!more) break;
+        }
+    }
+    store_acc(C, cre, cim, wave, lane);
+}
+
+// Schur step k inside a step group from split operands (BF16 precisions):
+// the pairs of k_schur, with L(next[a], k) in split slot slot0 + a and
+// U(k, next[b]) in slot slot0 + s + b, as the step's panel wrote them.
+template <bool THREE>
+__global__ void __launch_bounds__(256, 2) k_schur_split(float *__restrict__ tiles, int64_t mat_stride, int nt, int k,
+                                                       const int32_t *__restrict__ next, int s, int p,
+                                                       const int32_t *__restrict__ tile_id,
+                                                       const uint16_t *__restrict__ split, int64_t split_stride,
+                                                       int slot0) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
+    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int x = blockIdx.x;
+    int a, b;
+    if (x < p * s) {
+        a = x / s;
+        b = x % s;
+    } else {
+        const int y = x - p * s;
+        a = p + y / p;
+        b = y % p;
+    }
+    const uint16_t *sp = split + ((int64_t)m * split_stride + slot0) * SPLIT_TILE;
+    const u32x32 ra = fetch_split(sp + (int64_t)a * SPLIT_TILE, tid);
+    const u32x32 rb = fetch_split(sp + (int64_t)(s + b) * SPLIT_TILE, tid);
+    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)next[a] * nt + next[b]]);
+    floatx16 cre, cim;
+    load_acc(C, cre, cim, wave, lane);
+    put_split(As, ra, tid);
+    put_split(Bs, rb, tid);
+    __syncthreads();
+    mma_split<THREE>(As, Bs, cre, cim, wave, lane);
+    store_acc(C, cre, cim, wave, lane);
+}
+
+template __global__ void k_schur_split<true>(float *, int64_t, int, int, const int32_t *, int, int, const int32_t *,
+                                             const uint16_t *, int64_t, int);
+template __global__ void k_schur_split<false>(float *, int64_t, int, int, const int32_t *, int, int, const int32_t *,
+                                              const uint16_t *, int64_t, int);
+
+template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                                   const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                   const int32_t *);
+template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                                    const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                    const int32_t *);
 
 }  // namespace aosx

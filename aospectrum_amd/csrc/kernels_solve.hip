@@ -88,6 +88,64 @@ __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int
     }
 }
 
+// Trailing update of a step group [g0, g1) of a sweep: Y_i -= sum_k T_ik Y_k
+// over the group's steps k with tile T_ik present, for the rows i of the
+// group's list (rows the group's steps couple to, outside the group).  T_ik is
+// tile (i,k) or, with ctrans, the conjugate transpose of tile (k,i).  A block
+// keeps NC chunks of Y_i in the MFMA accumulators across the group's steps,
+// so Y_i is read and written once per group instead of once per step.  Grid:
+// 8 * ceil(nrows * nmat * ngrp / 8) blocks in one dimension, mapped to items
+// (matrix, chunk group, row) so that each XCD takes a contiguous run (rows of
+// one matrix share the group's Y_k blocks in its L2).
+template <int MODE, int NC>
+__global__ void __launch_bounds__(256) k_solve_trail(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
+                                                    const float *__restrict__ tiles, int64_t mat_stride,
+                                                    const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
+                                                    const int32_t *__restrict__ rows, int nrows, int nmat,
+                                                    int ctrans) {
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ngrp = (nchunk + NC - 1) / NC;
+    const int64_t total = (int64_t)nrows * nmat * ngrp, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int r = (int)(item % nrows);
+    const int64_t mc = item / nrows;
+    const int m = (int)(mc / ngrp), c0 = (int)(mc % ngrp) * NC;
+    const int i = rows[r];
+    const int nc = min(NC, nchunk - c0);
+    floatx16 cre[NC], cim[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        if (c < nc) load_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
+    for (int k = g0; k < g1; ++k) {
+        const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
+        if (id < 0) continue;
+        const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+        __syncthreads();  // the previous products are done with LDS
+        if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
+        else stage_a(t, Are, Aim, tid);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (c >= nc) break;
+            if (c > 0) __syncthreads();
+            stage_b(y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c), Bre, Bim, tid);
+            __syncthreads();
+            if (MODE == 0) mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
+            else mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        if (c < nc) store_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
+}
+
+template __global__ void k_solve_trail<0, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
+                                             int, const int32_t *, int, int, int);
+template __global__ void k_solve_trail<1, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
+                                             int, const int32_t *, int, int, int);
+
 // Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
 // (row-major n x w_src complex64); columns >= w_src and padded rows are zero.
 __global__ void k_pack_rhs(float *__restrict__ y, int64_t y_mat_stride, int nchunk, const float2 *__restrict__ src,

@@ -70,7 +70,8 @@ Bundle::~Bundle() {
         for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
                         (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
-                        (void *)d_atom_off})
+                        (void *)d_atom_off, (void *)d_pairs, (void *)d_ppos, (void *)d_step_s,
+                        (void *)d_fwd_rows, (void *)d_bwd_rows})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -169,7 +170,92 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     for (int64_t a = 0; a <= atoms; ++a) off[a] = (int32_t)atom_offsets[a];
     for (int64_t a = 0; a < atoms; ++a) max_atom = std::max<int>(max_atom, off[a + 1] - off[a]);
     d_atom_off = upload_vec(off, resident_bytes);
+    int group = 16;
+    if (const char *env = std::getenv("AOSX_SCHUR_GROUP")) group = std::min(32, std::max(1, std::atoi(env)));
+    plan_schur(group);
     peak_bytes = resident_bytes;
+}
+
+// Step groups of `group` consecutive elimination steps and the trailing pair
+// list of each: (i, j) with i, j >= g1 and i, j in next[t] for some step t of
+// the group.  Pairs are listed in 8 x 8 blocks of (i, j) so that the blocks
+// resident on one XCD at a time share 8 L rows and 8 U columns.
+void Bundle::plan_schur(int group) {
+    const int nt = ts.n_diag;
+    schur_group = group;
+    grp_begin.clear();
+    grp_pair_ptr.assign(1, 0);
+    grp_updates.clear();
+    std::vector<int32_t> pairs;
+    std::vector<uint8_t> mark((size_t)nt * nt, 0);
+    std::vector<int32_t> ppos((size_t)nt * nt, -1), step_s(nt, 0);
+    split_tiles = 0;
+    for (int g0 = 0; g0 < nt; g0 += group) {
+        const int g1 = std::min(nt, g0 + group);
+        grp_begin.push_back(g0);
+        double updates = 0.0;
+        int64_t slot = 0;
+        std::vector<int32_t> rows;  // tiles >= g1 touched by the group, ascending
+        for (int t = g0; t < g1; ++t) {
+            const int32_t *nx = ts.next_idx.data() + ts.next_ptr[t];
+            const int s = ts.next_ptr[t + 1] - ts.next_ptr[t];
+            step_s[t] = s;
+            for (int a = 0; a < s; ++a) ppos[(size_t)t * nt + nx[a]] = (int32_t)(slot + a);
+            slot += 2 * s;
+            int p = 0;
+            while (p < s && nx[p] < g1) ++p;
+            updates += (double)(s - p) * (s - p);
+            for (int a = p; a < s; ++a) {
+                rows.push_back(nx[a]);
+                for (int b = p; b < s; ++b) mark[(size_t)nx[a] * nt + nx[b]] = 1;
+            }
+        }
+        std::sort(rows.begin(), rows.end());
+        rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+        const int r = (int)rows.size();
+        for (int bi = 0; bi < r; bi += 8)
+            for (int bj = 0; bj < r; bj += 8)
+                for (int a = bi; a < std::min(r, bi + 8); ++a)
+                    for (int b = bj; b < std::min(r, bj + 8); ++b) {
+                        uint8_t &f = mark[(size_t)rows[a] * nt + rows[b]];
+                        if (!f) continue;
+                        f = 0;
+                        pairs.push_back(rows[a]);
+                        pairs.push_back(rows[b]);
+                    }
+        grp_pair_ptr.push_back((int64_t)pairs.size() / 2);
+        grp_updates.push_back(updates);
+        split_tiles = std::max(split_tiles, slot);
+    }
+    grp_begin.push_back(nt);
+    std::vector<int32_t> frows, brows;
+    fwd_row_ptr.assign(1, 0);
+    bwd_row_ptr.assign(1, 0);
+    for (size_t g = 0; g + 1 < grp_begin.size(); ++g) {
+        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+        std::vector<int32_t> f, b;
+        for (int k = g0; k < g1; ++k) {
+            for (int32_t a = ts.next_ptr[k]; a < ts.next_ptr[k + 1]; ++a)
+                if (ts.next_idx[a] >= g1) f.push_back(ts.next_idx[a]);
+            for (int32_t a = ts.prev_ptr[k]; a < ts.prev_ptr[k + 1]; ++a)
+                if (ts.prev_idx[a] < g0) b.push_back(ts.prev_idx[a]);
+        }
+        for (auto *v : {&f, &b}) {
+            std::sort(v->begin(), v->end());
+            v->erase(std::unique(v->begin(), v->end()), v->end());
+        }
+        frows.insert(frows.end(), f.begin(), f.end());
+        brows.insert(brows.end(), b.begin(), b.end());
+        fwd_row_ptr.push_back((int64_t)frows.size());
+        bwd_row_ptr.push_back((int64_t)brows.size());
+    }
+    for (int32_t *p : {d_pairs, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows})
+        if (p) AOSX_HIP(hipFree(p));
+    d_fwd_rows = upload_vec(frows, resident_bytes);
+    d_bwd_rows = upload_vec(brows, resident_bytes);
+    d_pairs = upload_vec(pairs, resident_bytes);
+    d_ppos = upload_vec(ppos, resident_bytes);
+    d_step_s = upload_vec(step_s, resident_bytes);
 }
 
 void Bundle::spmm(float2 *y, const float2 *a, const int32_t *d_kidx, const float2 *x, const int32_t *d_xidx, int w,
@@ -240,37 +326,74 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
     AOSX_LAUNCHED();
     const int nt = ts.n_diag;
     const float pivot_rel = 1.0e-6f;
-    for (int k = 0; k < nt; ++k) {
-        const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
-        const int32_t *nx = d_next + ts.next_ptr[k];
-        {
-            EventTimer tm(kstats.enabled, stream);
-            k_getrf_diag<<<nmat, 256, 0, stream>>>(tiles, mstride, dinv, nt, k, d_tile_id, d_replaced, pivot_rel);
-            AOSX_LAUNCHED();
-            if (kstats.enabled) { kstats.getrf_ms += tm.stop(); kstats.getrf_launches++; }
-        }
-        if (s == 0) continue;
-        {
-            EventTimer tm(kstats.enabled, stream);
-            k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id);
-            AOSX_LAUNCHED();
-            if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
-        }
-        {
-            EventTimer tm(kstats.enabled, stream);
-            dim3 grid(s * s, nmat);
-            if (precision == AOSX_PRECISION_BF16X3_IR)
-                k_schur<1><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
-            else if (precision == AOSX_PRECISION_BF16_IR)
-                k_schur<2><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
-            else
-                k_schur<0><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, d_tile_id);
-            AOSX_LAUNCHED();
-            if (kstats.enabled) {
-                kstats.schur_ms += tm.stop();
-                kstats.schur_launches++;
-                kstats.schur_flops += 8.0 * 64.0 * 64.0 * 64.0 * (double)s * s * nmat;
+    const int ng = (int)grp_begin.size() - 1;
+    const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
+    uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
+    for (int g = 0; g < ng; ++g) {
+        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+        int next_slot = 0;  // split slots are handed out step by step within the group
+        for (int k = g0; k < g1; ++k) {
+            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+            const int slot0 = next_slot;  // split slot of this step's first panel tile
+            next_slot += 2 * s;
+            const int32_t *nx_host = ts.next_idx.data() + ts.next_ptr[k];
+            const int32_t *nx = d_next + ts.next_ptr[k];
+            {
+                EventTimer tm(kstats.enabled, stream);
+                k_getrf_diag<<<nmat, 256, 0, stream>>>(tiles, mstride, dinv, nt, k, d_tile_id, d_replaced,
+                                                       pivot_rel);
+                AOSX_LAUNCHED();
+                if (kstats.enabled) { kstats.getrf_ms += tm.stop(); kstats.getrf_launches++; }
             }
+            if (s == 0) continue;
+            {
+                EventTimer tm(kstats.enabled, stream);
+                k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id, split,
+                                                               split_tiles, slot0);
+                AOSX_LAUNCHED();
+                if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
+            }
+            int p = 0;  // next[k][0..p) lie inside the group
+            while (p < s && nx_host[p] < g1) ++p;
+            if (p == 0) continue;
+            {
+                EventTimer tm(kstats.enabled, stream);
+                dim3 grid(p * s + (s - p) * p, nmat);
+                if (precision == AOSX_PRECISION_BF16X3_IR)
+                    k_schur_split<true><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id, split,
+                                                                  split_tiles, slot0);
+                else if (precision == AOSX_PRECISION_BF16_IR)
+                    k_schur_split<false><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id,
+                                                                   split, split_tiles, slot0);
+                else
+                    k_schur<<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id);
+                AOSX_LAUNCHED();
+                if (kstats.enabled) {
+                    kstats.inner_ms += tm.stop();
+                    kstats.inner_launches++;
+                    kstats.inner_flops += 8.0 * 64.0 * 64.0 * 64.0 * (double)(p * s + (s - p) * p) * nmat;
+                }
+            }
+        }
+        const int64_t np = grp_pair_ptr[g + 1] - grp_pair_ptr[g];
+        if (np == 0) continue;
+        EventTimer tm(kstats.enabled, stream);
+        const int64_t items = np * nmat;
+        const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
+        const int32_t *pr = d_pairs + 2 * grp_pair_ptr[g];
+        if (precision == AOSX_PRECISION_BF16X3_IR)
+            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat,
+                                                                  d_tile_id, split, split_tiles, d_ppos, d_step_s);
+        else if (precision == AOSX_PRECISION_BF16_IR)
+            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat,
+                                                                   d_tile_id, split, split_tiles, d_ppos, d_step_s);
+        else
+            k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat, d_tile_id);
+        AOSX_LAUNCHED();
+        if (kstats.enabled) {
+            kstats.schur_ms += tm.stop();
+            kstats.schur_launches++;
+            kstats.schur_flops += 8.0 * 64.0 * 64.0 * 64.0 * grp_updates[g] * nmat;
         }
     }
 }
@@ -285,26 +408,54 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
+    auto trail = split ? k_solve_trail<1, 2> : k_solve_trail<0, 2>;
+    const int ngrp = (nchunk + 1) / 2;
+    const int ng = (int)grp_begin.size() - 1;
+    auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
+        if (nrows == 0) return;
+        const int64_t items = nrows * nmat * ngrp;
+        trail<<<(unsigned)(8 * ((items + 7) / 8)), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
+                                                                    nt, g0, g1, rows, (int)nrows, nmat,
+                                                                    ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+    };
     EventTimer tm(kstats.enabled, stream);
-    for (int k = 0; k < nt; ++k) {
-        diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
-        AOSX_LAUNCHED();
-        const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
-        if (s) {
-            update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                      d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+    // forward: Y_k <- D_k Y_k, then the rows of the group below k, then the
+    // group's trailing rows in one launch
+    for (int g = 0; g < ng; ++g) {
+        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+        for (int k = g0; k < g1; ++k) {
+            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
             AOSX_LAUNCHED();
+            const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
+            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+            int p = 0;
+            while (p < s && nx[p] < g1) ++p;
+            if (p) {
+                update<<<dim3(p, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                          d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+                AOSX_LAUNCHED();
+            }
         }
+        trailing(g0, g1, d_fwd_rows + fwd_row_ptr[g], fwd_row_ptr[g + 1] - fwd_row_ptr[g]);
     }
-    for (int k = nt - 1; k >= 0; --k) {
-        diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
-        AOSX_LAUNCHED();
-        const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
-        if (s) {
-            update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                      d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
+    // backward: the same groups in reverse, steps descending
+    for (int g = ng - 1; g >= 0; --g) {
+        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+        for (int k = g1 - 1; k >= g0; --k) {
+            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
             AOSX_LAUNCHED();
+            const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
+            const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
+            int a = s;
+            while (a > 0 && pv[a - 1] >= g0) --a;
+            if (a < s) {
+                update<<<dim3(s - a, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_prev + ts.prev_ptr[k] + a, ctrans ? 1 : 0);
+                AOSX_LAUNCHED();
+            }
         }
+        trailing(g0, g1, d_bwd_rows + bwd_row_ptr[g], bwd_row_ptr[g + 1] - bwd_row_ptr[g]);
     }
     if (kstats.enabled) {
         kstats.solve_ms += tm.stop();

@@ -44,7 +44,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 
 # per-kernel table are in profiles/k_schur_bulk3d_traffic.md.  Counters
 # cannot be read from inside a timed run, so the figure is carried here and
 # refreshed whenever the kernel changes.
-MEASURED_SCHUR_TRAFFIC = {("bulk3d", "bf16x3_ir"): 57.27e9}
+MEASURED_SCHUR_TRAFFIC: dict = {}
 
 
 def log(*parts):
@@ -366,7 +366,7 @@ def main() -> int:
         "factor_flops_per_matrix": info["factor_flops"],
         "roofline": {
             "bound": "mfma",
-            "kernel": f"k_schur ({args.precision})",
+            "kernel": f"k_schur_group ({args.precision}, trailing update of each step group)",
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",

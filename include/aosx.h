@@ -286,8 +286,8 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *request,
 typedef struct {
     uint32_t abi_version;
     uint32_t struct_size;
-    int64_t schur_launches;
-    double schur_flops;            /* algorithmic real flops of the Schur GEMMs    */
+    int64_t schur_launches;        /* trailing Schur updates of the step groups    */
+    double schur_flops;            /* algorithmic real flops of those launches     */
     double schur_ms;
     int64_t panel_launches;
     double panel_ms;
@@ -299,6 +299,9 @@ typedef struct {
     int64_t assemble_launches;
     double assemble_ms;
     double assemble_bytes;         /* algorithmic bytes of the Bloch assembly      */
+    int64_t inner_launches;        /* Schur updates inside a step group            */
+    double inner_flops;
+    double inner_ms;
 } aosx_kernel_stats_v1;
 
 int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_stats_v1 *stats);
