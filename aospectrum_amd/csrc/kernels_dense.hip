@@ -43,13 +43,18 @@ __global__ void __launch_bounds__(256) k_spmm(float2 *__restrict__ y, const floa
 // the same column list (the pattern is a union of full atom blocks), so one
 // workgroup takes one atom's rows and reads each needed row of x once for all
 // of them -- an atom's worth (13x) less gather traffic than row-wise SpMM.
-// Values are staged through LDS in chunks of 64 columns; accumulation is FP64.
+// Values are staged through LDS in chunks of CH columns; accumulation is FP64.
 //   RESIDUAL: out[m] = b[b_index[m]] - (H_k - z_m S_k) x[m], with ||out||^2 and
 //             ||b||^2 added to sumsq[m], bsq[m] (FP64 iterative refinement);
 //   else    : out[m] = A_k x[x_index[m]] with A = h (H or S).
-// Wave q of the block owns rows q, q+4, ... of the atom (at most 8 rows each,
-// so atoms of up to 32 orbitals); lanes own columns.
-template <bool RESIDUAL>
+// The 8 half-waves of the block split the column list (half-wave q takes
+// entries q, q+8, ...) and each accumulates every row of the atom (up to MAXR)
+// for 32 columns of x, so each gathered x element is loaded once per block and
+// feeds all of the atom's rows; the partial sums are reduced through LDS.
+// Grid: 8 * ceil(n_atoms * nmat / 8) blocks in one dimension; block b takes
+// item (b % 8) * per + b / 8 = (matrix, atom) so that each XCD walks a
+// contiguous run of atoms of one matrix, whose gathers share x rows in L2.
+template <bool RESIDUAL, int MAXR>
 __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, const float2 *__restrict__ b,
                                                   const int32_t *__restrict__ b_index, const float2 *__restrict__ x,
                                                   const int32_t *__restrict__ x_index, int w,
@@ -58,12 +63,18 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
                                                   const int32_t *__restrict__ indptr,
                                                   const int32_t *__restrict__ indices,
                                                   const int32_t *__restrict__ atom_off, int64_t n, int64_t nnz,
-                                                  double *__restrict__ sumsq, double *__restrict__ bsq) {
-    constexpr int CH = 64, MAXR = 32;
+                                                  int n_atoms, int nmat, double *__restrict__ sumsq,
+                                                  double *__restrict__ bsq) {
+    constexpr int CH = 64, NS = 8;
     __shared__ double2 vals[CH][MAXR];
+    __shared__ double2 red[MAXR][32];
     __shared__ int32_t cidx[CH];
-    const int a = blockIdx.x, m = blockIdx.y;
+    const int64_t total = (int64_t)n_atoms * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = (int)(item / n_atoms), a = (int)(item % n_atoms);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, cl = lane & 31, stream = wave * 2 + half;
     const int r0 = atom_off[a], na = atom_off[a + 1] - r0;
     const int p0 = indptr[r0], len = indptr[r0 + 1] - p0;
     const int k = kidx[m];
@@ -73,11 +84,12 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
     const float2 *xm = x + (int64_t)(RESIDUAL ? m : x_index[m]) * n * w;
     float2 *om = out + (int64_t)m * n * w;
     double local = 0.0, blocal = 0.0;
-    for (int c0 = 0; c0 < w; c0 += 64) {
-        const int c = c0 + lane;
-        double accr[8], acci[8];
+    for (int c0 = 0; c0 < w; c0 += 32) {
+        const int c = c0 + cl;
+        const bool cv = c < w;
+        double ar[MAXR], ai[MAXR];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) accr[q] = acci[q] = 0.0;
+        for (int r = 0; r < MAXR; ++r) ar[r] = ai[r] = 0.0;
         for (int q0 = 0; q0 < len; q0 += CH) {
             const int cnt = min(CH, len - q0);
             __syncthreads();
@@ -95,41 +107,69 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
             }
             for (int e = tid; e < cnt; e += 256) cidx[e] = indices[p0 + q0 + e];
             __syncthreads();
-            if (c < w) {
-                for (int pp = 0; pp < cnt; ++pp) {
-                    const float2 xv = xm[(int64_t)cidx[pp] * w + c];
+            float2 xv[CH / NS];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const int r = wave + 4 * q;
-                        if (r < na) {
-                            const double2 v = vals[pp][r];
-                            accr[q] += v.x * xv.x - v.y * xv.y;
-                            acci[q] += v.x * xv.y + v.y * xv.x;
-                        }
+            for (int u = 0; u < CH / NS; ++u) {
+                const int pp = stream + NS * u;
+                xv[u] = (cv && pp < cnt) ? xm[(int64_t)cidx[pp] * w + c] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < CH / NS; ++u) {
+                const int pp = stream + NS * u;
+                if (pp >= cnt) break;
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r) {
+                    if (r < na) {
+                        const double2 v = vals[pp][r];
+                        ar[r] += v.x * xv[u].x - v.y * xv[u].y;
+                        ai[r] += v.x * xv[u].y + v.y * xv[u].x;
                     }
                 }
             }
         }
-        if (c < w) {
+        // half-waves: lanes 32..63 into 0..31
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int r = wave + 4 * q;
-                if (r >= na) continue;
+        for (int r = 0; r < MAXR; ++r) {
+            ar[r] += __shfl_down(ar[r], 32);
+            ai[r] += __shfl_down(ai[r], 32);
+        }
+        // waves 1..3 into wave 0
+        for (int src = 1; src < 4; ++src) {
+            __syncthreads();
+            if (wave == src && half == 0) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (r < na) red[r][cl] = make_double2(ar[r], ai[r]);
+            }
+            __syncthreads();
+            if (wave == 0 && half == 0) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (r < na) {
+                        ar[r] += red[r][cl].x;
+                        ai[r] += red[r][cl].y;
+                    }
+            }
+        }
+        if (wave == 0 && half == 0 && cv) {
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                if (r >= na) break;
                 const int64_t row = r0 + r;
                 if (RESIDUAL) {
                     const float2 bv = b[(int64_t)b_index[m] * n * w + row * w + c];
-                    const double rr = bv.x - accr[q], ri = bv.y - acci[q];
+                    const double rr = bv.x - ar[r], ri = bv.y - ai[r];
                     om[row * w + c] = make_float2((float)rr, (float)ri);
                     local += rr * rr + ri * ri;
                     blocal += (double)bv.x * bv.x + (double)bv.y * bv.y;
                 } else {
-                    om[row * w + c] = make_float2((float)accr[q], (float)acci[q]);
+                    om[row * w + c] = make_float2((float)ar[r], (float)ai[r]);
                 }
             }
         }
     }
-    if (RESIDUAL) {
-        for (int off = 32; off > 0; off >>= 1) {
+    if (RESIDUAL && wave == 0) {
+        for (int off = 16; off > 0; off >>= 1) {
             local += __shfl_down(local, off);
             blocal += __shfl_down(blocal, off);
         }

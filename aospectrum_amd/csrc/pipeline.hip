@@ -261,9 +261,10 @@ void Bundle::plan_schur(int group) {
 void Bundle::spmm(float2 *y, const float2 *a, const int32_t *d_kidx, const float2 *x, const int32_t *d_xidx, int w,
                   int nmat) {
     if (max_atom <= 32) {
-        k_atom_spmm<false><<<dim3((unsigned)n_atoms, nmat), 256, 0, stream>>>(
-            y, nullptr, nullptr, x, d_xidx, w, a, nullptr, d_kidx, nullptr, d_indptr, d_indices, d_atom_off, n, nnz,
-            nullptr, nullptr);
+        const unsigned blocks = (unsigned)(8 * ((n_atoms * nmat + 7) / 8));
+        auto kern = max_atom <= 16 ? k_atom_spmm<false, 16> : k_atom_spmm<false, 32>;
+        kern<<<blocks, 256, 0, stream>>>(y, nullptr, nullptr, x, d_xidx, w, a, nullptr, d_kidx, nullptr, d_indptr,
+                                         d_indices, d_atom_off, n, nnz, (int)n_atoms, nmat, nullptr, nullptr);
     } else {
         k_spmm<<<dim3(blocks_for(n, 4, 2048), nmat), 256, 0, stream>>>(y, a, d_kidx, x, d_xidx, w, d_indptr,
                                                                        d_indices, n, nnz);
@@ -275,8 +276,10 @@ void Bundle::residual(float2 *r, const float2 *b, const int32_t *d_bidx, const f
                       const float2 *d_s, const int32_t *d_kidx, const float2 *d_z, int nmat, double *sumsq,
                       double *bsq) {
     if (max_atom <= 32) {
-        k_atom_spmm<true><<<dim3((unsigned)n_atoms, nmat), 256, 0, stream>>>(
-            r, b, d_bidx, x, nullptr, w, d_h, d_s, d_kidx, d_z, d_indptr, d_indices, d_atom_off, n, nnz, sumsq, bsq);
+        const unsigned blocks = (unsigned)(8 * ((n_atoms * nmat + 7) / 8));
+        auto kern = max_atom <= 16 ? k_atom_spmm<true, 16> : k_atom_spmm<true, 32>;
+        kern<<<blocks, 256, 0, stream>>>(r, b, d_bidx, x, nullptr, w, d_h, d_s, d_kidx, d_z, d_indptr, d_indices,
+                                         d_atom_off, n, nnz, (int)n_atoms, nmat, sumsq, bsq);
     } else {
         k_residual_shifted<<<dim3(blocks_for(n, 4, 2048), nmat), 256, 0, stream>>>(
             r, b, d_bidx, x, w, d_h, d_s, d_kidx, d_z, d_indptr, d_indices, n, nnz, sumsq, bsq);
