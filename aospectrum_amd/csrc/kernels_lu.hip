@@ -517,6 +517,51 @@ __device__ __forceinline__ void mma_split(const uint16_t *As, const uint16_t *Bs
     }
 }
 
+// Stages an FP32 planar tile G (64 x 64 complex, row-major) into LDS as a
+// split operand S (planes re_hi, re_lo, im_hi, im_lo; row stride SLD) whose
+// rows are the MFMA's M or N index and whose columns are its K index:
+//   transpose = false: S[r][c] = f(G[r][c])   (G contracted over its columns)
+//   transpose = true : S[r][c] = f(G[c][r])   (G contracted over its rows)
+// with f = negation when `negate`, complex conjugation when `conj`.  Thread
+// tid converts the 4 x 4 block (rows 4*(tid/16).., columns 4*(tid%16)..) and
+// writes 4 runs of 4 consecutive K values per plane.
+__device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_t *S, bool transpose, bool negate,
+                                            bool conj, int tid) {
+    const int br = tid >> 4, bc = tid & 15;
+    const float4 *g4 = reinterpret_cast<const float4 *>(g);
+    float re[4][4], im[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 vr = g4[(4 * br + q) * (TILE / 4) + bc];
+        const float4 vi = g4[PLANE / 4 + (4 * br + q) * (TILE / 4) + bc];
+        re[q][0] = vr.x; re[q][1] = vr.y; re[q][2] = vr.z; re[q][3] = vr.w;
+        im[q][0] = vi.x; im[q][1] = vi.y; im[q][2] = vi.z; im[q][3] = vi.w;
+    }
+    const float sr = negate ? -1.f : 1.f, si = (negate != conj) ? -1.f : 1.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        // run u: S row and its 4 consecutive K values
+        const int row = transpose ? 4 * bc + u : 4 * br + u;
+        const int k0 = transpose ? 4 * br : 4 * bc;
+        uint16_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float xr = sr * (transpose ? re[e][u] : re[u][e]);
+            const float xi = si * (transpose ? im[e][u] : im[u][e]);
+            split_pair(xr, rh[e], rl[e]);
+            split_pair(xi, ih[e], il[e]);
+        }
+        const int off = row * SLD + k0;
+        *reinterpret_cast<uint2 *>(S + off) = make_uint2(rh[0] | ((uint32_t)rh[1] << 16), rh[2] | ((uint32_t)rh[3] << 16));
+        *reinterpret_cast<uint2 *>(S + SPL + off) =
+            make_uint2(rl[0] | ((uint32_t)rl[1] << 16), rl[2] | ((uint32_t)rl[3] << 16));
+        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) =
+            make_uint2(ih[0] | ((uint32_t)ih[1] << 16), ih[2] | ((uint32_t)ih[3] << 16));
+        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) =
+            make_uint2(il[0] | ((uint32_t)il[1] << 16), il[2] | ((uint32_t)il[3] << 16));
+    }
+}
+
 // One split tile = 2048 chunks of 16 B; thread tid moves chunks q * 256 + tid.
 // The chunks travel in a register vector (not an array, which the compiler
 // may place in scratch memory when it is carried across loop iterations).

@@ -29,10 +29,7 @@ __device__ __forceinline__ void stage_a_ctrans(const float *__restrict__ g, floa
 }
 
 // Y_k <- D_k Y_k for all chunks; D_k = inv(L_kk) (which 0) or inv(U_kk)
-// (which 1), conjugate-transposed when ctrans.  MODE 0 multiplies with FP32
-// MFMA, MODE 1 with the three-product BF16 split (refinement on FP64
-// residuals removes the difference).
-template <int MODE>
+// (which 1), conjugate-transposed when ctrans.  FP32 MFMA.
 __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                    const float *__restrict__ dinv, int nt, int k, int which,
                                                    int ctrans) {
@@ -48,20 +45,34 @@ __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64
         stage_b(yk, Bre, Bim, tid);
         __syncthreads();
         floatx16 cre = {}, cim = {};
-        if (MODE == 0) {
-            mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
-        } else {
-            mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
-            cre = -cre;
-            cim = -cim;
-        }
+        mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        store_acc(yk, cre, cim, wave, lane);
+    }
+}
+
+// The same with both operands split into BF16 hi + lo parts (three MFMA
+// products, FP32 accumulation; refinement on FP64 residuals removes the
+// difference): the BF16 precisions.
+__global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
+                                                            const float *__restrict__ dinv, int nt, int k, int which,
+                                                            int ctrans) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
+    const int m = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+    stage_split(d, As, ctrans != 0, false, ctrans != 0, tid);
+    for (int c = 0; c < nchunk; ++c) {
+        float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
+        __syncthreads();
+        stage_split(yk, Bs, true, false, false, tid);
+        __syncthreads();
+        floatx16 cre = {}, cim = {};
+        mma_split<true>(As, Bs, cre, cim, wave, lane);
         store_acc(yk, cre, cim, wave, lane);
     }
 }
 
 // Y_i -= T Y_k for i = list[blockIdx.x], all chunks; T is tile (i,k) or,
-// with ctrans, the conjugate transpose of tile (k,i).  MODE as k_solve_diag.
-template <int MODE>
+// with ctrans, the conjugate transpose of tile (k,i).  FP32 MFMA.
 __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                      const float *__restrict__ tiles, int64_t mat_stride,
                                                      const int32_t *__restrict__ tile_id, int nt, int k,
@@ -82,8 +93,33 @@ __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int
         floatx16 cre, cim;
         load_acc(yi, cre, cim, wave, lane);
         __syncthreads();
-        if (MODE == 0) mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
-        else mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        store_acc(yi, cre, cim, wave, lane);
+    }
+}
+
+// The same with split operands (BF16 precisions): -T (or -T^H) and Y_k staged
+// as split tiles.
+__global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict__ y, int64_t y_mat_stride,
+                                                              int nchunk, const float *__restrict__ tiles,
+                                                              int64_t mat_stride, const int32_t *__restrict__ tile_id,
+                                                              int nt, int k, const int32_t *__restrict__ list,
+                                                              int ctrans) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
+    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int i = list[blockIdx.x];
+    const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
+    const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+    stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
+    for (int c = 0; c < nchunk; ++c) {
+        const float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
+        float *yi = y + rhs_block(y_mat_stride, m, i, nchunk, c);
+        __syncthreads();
+        stage_split(yk, Bs, true, false, false, tid);
+        floatx16 cre, cim;
+        load_acc(yi, cre, cim, wave, lane);
+        __syncthreads();
+        mma_split<true>(As, Bs, cre, cim, wave, lane);
         store_acc(yi, cre, cim, wave, lane);
     }
 }
@@ -97,14 +133,17 @@ __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int
 // 8 * ceil(nrows * nmat * ngrp / 8) blocks in one dimension, mapped to items
 // (matrix, chunk group, row) so that each XCD takes a contiguous run (rows of
 // one matrix share the group's Y_k blocks in its L2).
-template <int MODE, int NC>
-__global__ void __launch_bounds__(256) k_solve_trail(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
-                                                    const float *__restrict__ tiles, int64_t mat_stride,
-                                                    const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
-                                                    const int32_t *__restrict__ rows, int nrows, int nmat,
-                                                    int ctrans) {
-    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
-    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+template <bool SPLIT, int NC>
+__global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
+                                                       const float *__restrict__ tiles, int64_t mat_stride,
+                                                       const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
+                                                       const int32_t *__restrict__ rows, int nrows, int nmat,
+                                                       int ctrans) {
+    // FP32 operands (Are/Aim/Bre/Bim) or split operands (As/Bs) share one
+    // LDS area.
+    __shared__ __attribute__((aligned(16))) float lds[SPLIT ? (8 * SPL) / 2 : 2 * TILE * LDA + 2 * TILE2];
+    float *Are = lds, *Aim = lds + TILE * LDA, *Bre = lds + 2 * TILE * LDA, *Bim = Bre + TILE2;
+    uint16_t *As = reinterpret_cast<uint16_t *>(lds), *Bs = As + 4 * SPL;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ngrp = (nchunk + NC - 1) / NC;
     const int64_t total = (int64_t)nrows * nmat * ngrp, per = (total + 7) / 8;
@@ -124,16 +163,19 @@ __global__ void __launch_bounds__(256) k_solve_trail(float *__restrict__ y, int6
         if (id < 0) continue;
         const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
         __syncthreads();  // the previous products are done with LDS
-        if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
+        if (SPLIT) stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
+        else if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
         else stage_a(t, Are, Aim, tid);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             if (c >= nc) break;
             if (c > 0) __syncthreads();
-            stage_b(y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c), Bre, Bim, tid);
+            const float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c);
+            if (SPLIT) stage_split(yk, Bs, true, false, false, tid);
+            else stage_b(yk, Bre, Bim, tid);
             __syncthreads();
-            if (MODE == 0) mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
-            else mma_bf16_sub<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
+            if (SPLIT) mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
+            else mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
         }
     }
 #pragma unroll
@@ -141,10 +183,10 @@ __global__ void __launch_bounds__(256) k_solve_trail(float *__restrict__ y, int6
         if (c < nc) store_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
 }
 
-template __global__ void k_solve_trail<0, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
-                                             int, const int32_t *, int, int, int);
-template __global__ void k_solve_trail<1, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
-                                             int, const int32_t *, int, int, int);
+template __global__ void k_solve_trail<false, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
+                                                 int, int, const int32_t *, int, int, int);
+template __global__ void k_solve_trail<true, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
+                                                int, int, const int32_t *, int, int, int);
 
 // Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
 // (row-major n x w_src complex64); columns >= w_src and padded rows are zero.

@@ -405,13 +405,13 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
 // y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
 void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision) {
     const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
-    auto diag = split ? k_solve_diag<1> : k_solve_diag<0>;
-    auto update = split ? k_solve_update<1> : k_solve_update<0>;
+    auto diag = split ? k_solve_diag_split : k_solve_diag;
+    auto update = split ? k_solve_update_split : k_solve_update;
     float *y = reinterpret_cast<float *>(y2);
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
-    auto trail = split ? k_solve_trail<1, 2> : k_solve_trail<0, 2>;
+    auto trail = split ? k_solve_trail<true, 2> : k_solve_trail<false, 2>;
     const int ngrp = (nchunk + 1) / 2;
     const int ng = (int)grp_begin.size() - 1;
     auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
