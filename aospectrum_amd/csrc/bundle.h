@@ -100,6 +100,11 @@ struct Bundle {
     // Solve sweeps use the same step groups: rows outside group g that its
     // steps couple to, forward (i >= g1) and backward (i < g0), ascending.
     std::vector<int64_t> fwd_row_ptr, bwd_row_ptr;
+    // Per step t: the tiles of row and column t -- (t,t), (i,t), (t,i) for i in
+    // next[t] -- which take the updates of the group's earlier steps in one
+    // launch before t is factored (d_pairs[crout_ptr[t] .. crout_ptr[t+1])).
+    std::vector<int64_t> crout_ptr;
+    std::vector<double> crout_updates;  // tile updates of each of those launches
     int32_t *d_fwd_rows = nullptr, *d_bwd_rows = nullptr;
     int64_t n_atoms = 0;
     int max_atom = 0;                // largest atom, in orbitals

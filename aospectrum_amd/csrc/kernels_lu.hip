@@ -6,19 +6,19 @@
 // tile structure comes from the host symbolic analysis; for step k the tiles
 // next[k] hold L(i,k) and, by symmetry of the pattern, U(k,i).
 //
-// Steps are taken in groups of consecutive steps [g0, g1) (right-looking
-// across groups).  Per step k of a group:
+// Steps are taken in groups of consecutive steps [g0, g1): left-looking
+// inside a group, right-looking across groups.  Per step k of a group:
+//   crout  : A(i,j) -= sum_{t in [g0, k)} L(i,t) * U(t,j) for the tiles of
+//            row and column k ((k,k), (i,k), (k,i), i in next[k])        (MFMA)
 //   getrf  : factor the diagonal tile without row exchanges; a pivot below
 //            1e-6 * max|tile| is replaced by that bound (static pivoting in the
 //            sense of Li & Demmel's GESP, corrected later by refinement);
 //            also writes inv(L_kk) (unit lower) and inv(U_kk).
 //   panel  : L(i,k) <- L(i,k) * inv(U_kk),  U(k,i) <- inv(L_kk) * U(k,i)   (MFMA)
-//   schur  : A(i,j) -= L(i,k) * U(k,j) for i, j in next[k] with i < g1 or
-//            j < g1 (the tiles the group's later steps need)              (MFMA)
 // and after the group's last step one trailing launch applies
-//   A(i,j) -= sum_{k in group} L(i,k) * U(k,j)   for i, j >= g1,
-// keeping A(i,j) in the accumulators, so each trailing target is read and
-// written once per group.
+//   A(i,j) -= sum_{k in group} L(i,k) * U(k,j)   for i, j >= g1.
+// Both update launches keep the target in the MFMA accumulators across the
+// steps they apply, so every target is read and written once per group.
 // The complex products use four real FP32 MFMA products (v_mfma_f32_32x32x2_f32),
 // i.e. 8 real flops per complex multiply-add.  In the BF16 precisions the
 // panels also write their tiles split into hi + lo BF16 parts, which the Schur
@@ -229,58 +229,6 @@ __device__ __forceinline__ void mma_fp32(const float *Are, const float *Aim, con
     }
 }
 
-__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
-
-// acc -= A * B with every operand split as x = hi + lo (both BF16):
-// A*B ~ Ahi*Bhi + Ahi*Blo + Alo*Bhi (three BF16 MFMA products, FP32 accumulate);
-// THREE = false keeps only Ahi*Bhi.
-template <bool THREE>
-__device__ __forceinline__ void mma_bf16_sub(const float *Are, const float *Aim, const float *Bre,
-                                             const float *Bim, floatx16 &cre, floatx16 &cim, int wave,
-                                             int lane) {
-    const int i = (wave >> 1) * 32 + (lane & 31);
-    const int j = (wave & 1) * 32 + (lane & 31);
-    const int kb = (lane >> 5) * 8;
-#pragma unroll 1
-    for (int k0 = 0; k0 < TILE; k0 += 16) {
-        bf16x8 arh, arl, aih, ail, nrh, nrl, brh, brl, bih, bil, nih, nil;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = k0 + kb + e;
-            const float ar = Are[i * LDA + k], ai = Aim[i * LDA + k];
-            const float br = Bre[k * TILE + j], bi = Bim[k * TILE + j];
-            const __bf16 arH = to_bf16(ar), aiH = to_bf16(ai), brH = to_bf16(br), biH = to_bf16(bi);
-            arh[e] = arH; aih[e] = aiH; brh[e] = brH; bih[e] = biH;
-            nrh[e] = to_bf16(-ar);
-            nih[e] = to_bf16(-ai);
-            if (THREE) {
-                const __bf16 arL = to_bf16(ar - (float)arH), aiL = to_bf16(ai - (float)aiH);
-                arl[e] = arL; ail[e] = aiL;
-                nrl[e] = to_bf16(-(ar - (float)arH));
-                nil[e] = to_bf16(-(ai - (float)aiH));
-                brl[e] = to_bf16(br - (float)brH);
-                bil[e] = to_bf16(bi - (float)biH);
-            }
-        }
-        // re -= ar*br - ai*bi  ->  (-ar)*br + ai*bi
-        // im -= ar*bi + ai*br  ->  (-ar)*bi + (-ai)*br
-        cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrh, brh, cre, 0, 0, 0);
-        cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, bih, cre, 0, 0, 0);
-        cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrh, bih, cim, 0, 0, 0);
-        cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, brh, cim, 0, 0, 0);
-        if (THREE) {
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrh, brl, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrl, brh, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, bil, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, bih, cre, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrh, bil, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nrl, bih, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, brl, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nil, brh, cim, 0, 0, 0);
-        }
-    }
-}
-
 // Accumulator <-> tile element mapping of v_mfma_f32_32x32x*: register r of
 // lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32 of the wave's block.
 __device__ __forceinline__ void load_acc(const float *__restrict__ t, floatx16 &cre, floatx16 &cim, int wave,
@@ -397,46 +345,11 @@ __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_
     }
 }
 
-// Schur step k inside a step group: A(i,j) -= L(i,k) U(k,j) for the pairs
-// (next[a], next[b]) with a < p or b < p, where next[0..p) are the tiles of
-// the group (the rest take this step's update in the group's trailing launch).
-// Block x < p*s is pair (x / s, x % s); the (s-p)*p blocks after it are
-// (p + y / p, y % p).  With p == s this is the whole s*s Schur complement.
-// FP32 MFMA (the BF16 precisions use k_schur_split).
-__global__ void __launch_bounds__(256) k_schur(float *__restrict__ tiles, int64_t mat_stride, int nt, int k,
-                                              const int32_t *__restrict__ next, int s, int p,
-                                              const int32_t *__restrict__ tile_id) {
-    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
-    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
-    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int x = blockIdx.x;
-    int a, b;
-    if (x < p * s) {
-        a = x / s;
-        b = x % s;
-    } else {
-        const int y = x - p * s;
-        a = p + y / p;
-        b = y % p;
-    }
-    const int i = next[a], j = next[b];
-    const float *L = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + k]);
-    const float *U = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)k * nt + j]);
-    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
-    stage_a(L, Are, Aim, tid);
-    stage_b(U, Bre, Bim, tid);
-    floatx16 cre, cim;
-    load_acc(C, cre, cim, wave, lane);
-    __syncthreads();
-    mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
-    store_acc(C, cre, cim, wave, lane);
-}
-
-// Trailing update of a step group [g0, g1): A(i,j) -= sum_t L(i,t) U(t,j) over
-// the steps t of the group where both tiles exist, for every pair (i, j) of
-// the group's list (i, j >= g1).  The target stays in the MFMA accumulators
-// across the group's steps, so it is read and written once per group rather
-// than once per step.  The grid is 8 * ceil(npairs * nmat / 8) blocks in one
+// Multi-step update: A(i,j) -= sum_t L(i,t) U(t,j) over the steps t in
+// [g0, g1) where both tiles exist, for every pair (i, j) of the list (a
+// group's trailing targets, or the row/column targets of one step with the
+// group's earlier steps).  The target stays in the MFMA accumulators across
+// the steps, so it is read and written once rather than once per step.  The grid is 8 * ceil(npairs * nmat / 8) blocks in one
 // dimension; consecutive blocks go round-robin to the 8 XCDs, so block b works
 // on item (b % 8) * per + b / 8 and each XCD (with its own L2) takes one
 // contiguous run of the (matrix, pair) items, whose pairs share L rows and U
@@ -467,8 +380,8 @@ __global__ void __launch_bounds__(256) k_schur_group(float *__restrict__ tiles, 
     store_acc(C, cre, cim, wave, lane);
 }
 
-// Trailing update of a step group from split operands (BF16 precisions):
-// the same pairs and step loop as k_schur_group, with L and U read from the
+// Multi-step update from split operands (BF16 precisions): the same pairs
+// and step loop as k_schur_group, with L and U read from the
 // split tiles the panels of the group wrote.  Slot of L(i,t) in the matrix's
 // split area: ppos[t * nt + i] (-1 when absent); slot of U(t,j):
 // ppos[t * nt + j] + step_s[t].  The next step's two split tiles are loaded
@@ -633,45 +546,6 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     }
     store_acc(C, cre, cim, wave, lane);
 }
-
-// Schur step k inside a step group from split operands (BF16 precisions):
-// the pairs of k_schur, with L(next[a], k) in split slot slot0 + a and
-// U(k, next[b]) in slot slot0 + s + b, as the step's panel wrote them.
-template <bool THREE>
-__global__ void __launch_bounds__(256, 2) k_schur_split(float *__restrict__ tiles, int64_t mat_stride, int nt, int k,
-                                                       const int32_t *__restrict__ next, int s, int p,
-                                                       const int32_t *__restrict__ tile_id,
-                                                       const uint16_t *__restrict__ split, int64_t split_stride,
-                                                       int slot0) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
-    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int x = blockIdx.x;
-    int a, b;
-    if (x < p * s) {
-        a = x / s;
-        b = x % s;
-    } else {
-        const int y = x - p * s;
-        a = p + y / p;
-        b = y % p;
-    }
-    const uint16_t *sp = split + ((int64_t)m * split_stride + slot0) * SPLIT_TILE;
-    const u32x32 ra = fetch_split(sp + (int64_t)a * SPLIT_TILE, tid);
-    const u32x32 rb = fetch_split(sp + (int64_t)(s + b) * SPLIT_TILE, tid);
-    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)next[a] * nt + next[b]]);
-    floatx16 cre, cim;
-    load_acc(C, cre, cim, wave, lane);
-    put_split(As, ra, tid);
-    put_split(Bs, rb, tid);
-    __syncthreads();
-    mma_split<THREE>(As, Bs, cre, cim, wave, lane);
-    store_acc(C, cre, cim, wave, lane);
-}
-
-template __global__ void k_schur_split<true>(float *, int64_t, int, int, const int32_t *, int, int, const int32_t *,
-                                             const uint16_t *, int64_t, int);
-template __global__ void k_schur_split<false>(float *, int64_t, int, int, const int32_t *, int, int, const int32_t *,
-                                              const uint16_t *, int64_t, int);
 
 template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
                                                    const int32_t *, const uint16_t *, int64_t, const int32_t *,

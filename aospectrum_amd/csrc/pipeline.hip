@@ -228,6 +228,34 @@ void Bundle::plan_schur(int group) {
         split_tiles = std::max(split_tiles, slot);
     }
     grp_begin.push_back(nt);
+    // row/column targets of each step, appended after the trailing pairs
+    crout_ptr.assign(nt + 1, 0);
+    crout_updates.assign(nt, 0.0);
+    for (size_t g = 0; g + 1 < grp_begin.size(); ++g) {
+        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+        for (int t = g0; t < g1; ++t) {
+            crout_ptr[t] = (int64_t)pairs.size() / 2;
+            const int32_t *nx = ts.next_idx.data() + ts.next_ptr[t];
+            const int s = ts.next_ptr[t + 1] - ts.next_ptr[t];
+            auto has = [&](int tp, int i) {  // i in next[tp]
+                const int32_t *b = ts.next_idx.data() + ts.next_ptr[tp], *e = ts.next_idx.data() + ts.next_ptr[tp + 1];
+                return std::binary_search(b, e, i);
+            };
+            auto add = [&](int i, int j) {
+                int cnt = 0;
+                for (int tp = g0; tp < t; ++tp)
+                    if (has(tp, i) && has(tp, j)) ++cnt;
+                if (cnt == 0) return;
+                pairs.push_back(i);
+                pairs.push_back(j);
+                crout_updates[t] += cnt;
+            };
+            add(t, t);
+            for (int a = 0; a < s; ++a) add(nx[a], t);
+            for (int a = 0; a < s; ++a) add(t, nx[a]);
+        }
+    }
+    crout_ptr[nt] = (int64_t)pairs.size() / 2;
     std::vector<int32_t> frows, brows;
     fwd_row_ptr.assign(1, 0);
     bwd_row_ptr.assign(1, 0);
@@ -332,6 +360,22 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
     const int ng = (int)grp_begin.size() - 1;
     const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
     uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
+    // A(i,j) -= sum_{t in [t0, t1)} L(i,t) U(t,j) for the listed pairs, the
+    // target held in the accumulators across the steps
+    auto multi_update = [&](int t0, int t1, int64_t first, int64_t np) {
+        const int64_t items = np * nmat;
+        const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
+        const int32_t *pr = d_pairs + 2 * first;
+        if (precision == AOSX_PRECISION_BF16X3_IR)
+            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat,
+                                                                  d_tile_id, split, split_tiles, d_ppos, d_step_s);
+        else if (precision == AOSX_PRECISION_BF16_IR)
+            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat,
+                                                                   d_tile_id, split, split_tiles, d_ppos, d_step_s);
+        else
+            k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat, d_tile_id);
+        AOSX_LAUNCHED();
+    };
     for (int g = 0; g < ng; ++g) {
         const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
         int next_slot = 0;  // split slots are handed out step by step within the group
@@ -339,8 +383,18 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
             const int slot0 = next_slot;  // split slot of this step's first panel tile
             next_slot += 2 * s;
-            const int32_t *nx_host = ts.next_idx.data() + ts.next_ptr[k];
             const int32_t *nx = d_next + ts.next_ptr[k];
+            // row and column k take the updates of the group's earlier steps
+            const int64_t nc = crout_ptr[k + 1] - crout_ptr[k];
+            if (nc > 0) {
+                EventTimer tm(kstats.enabled, stream);
+                multi_update(g0, k, crout_ptr[k], nc);
+                if (kstats.enabled) {
+                    kstats.inner_ms += tm.stop();
+                    kstats.inner_launches++;
+                    kstats.inner_flops += 8.0 * 64.0 * 64.0 * 64.0 * crout_updates[k] * nmat;
+                }
+            }
             {
                 EventTimer tm(kstats.enabled, stream);
                 k_getrf_diag<<<nmat, 256, 0, stream>>>(tiles, mstride, dinv, nt, k, d_tile_id, d_replaced,
@@ -349,50 +403,17 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
                 if (kstats.enabled) { kstats.getrf_ms += tm.stop(); kstats.getrf_launches++; }
             }
             if (s == 0) continue;
-            {
-                EventTimer tm(kstats.enabled, stream);
-                k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id, split,
-                                                               split_tiles, slot0);
-                AOSX_LAUNCHED();
-                if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
-            }
-            int p = 0;  // next[k][0..p) lie inside the group
-            while (p < s && nx_host[p] < g1) ++p;
-            if (p == 0) continue;
-            {
-                EventTimer tm(kstats.enabled, stream);
-                dim3 grid(p * s + (s - p) * p, nmat);
-                if (precision == AOSX_PRECISION_BF16X3_IR)
-                    k_schur_split<true><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id, split,
-                                                                  split_tiles, slot0);
-                else if (precision == AOSX_PRECISION_BF16_IR)
-                    k_schur_split<false><<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id,
-                                                                   split, split_tiles, slot0);
-                else
-                    k_schur<<<grid, 256, 0, stream>>>(tiles, mstride, nt, k, nx, s, p, d_tile_id);
-                AOSX_LAUNCHED();
-                if (kstats.enabled) {
-                    kstats.inner_ms += tm.stop();
-                    kstats.inner_launches++;
-                    kstats.inner_flops += 8.0 * 64.0 * 64.0 * 64.0 * (double)(p * s + (s - p) * p) * nmat;
-                }
-            }
+            EventTimer tm(kstats.enabled, stream);
+            k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id, split,
+                                                           split_tiles, slot0);
+            AOSX_LAUNCHED();
+            if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
         }
+        // every target outside the group's rows and columns, once
         const int64_t np = grp_pair_ptr[g + 1] - grp_pair_ptr[g];
         if (np == 0) continue;
         EventTimer tm(kstats.enabled, stream);
-        const int64_t items = np * nmat;
-        const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
-        const int32_t *pr = d_pairs + 2 * grp_pair_ptr[g];
-        if (precision == AOSX_PRECISION_BF16X3_IR)
-            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat,
-                                                                  d_tile_id, split, split_tiles, d_ppos, d_step_s);
-        else if (precision == AOSX_PRECISION_BF16_IR)
-            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat,
-                                                                   d_tile_id, split, split_tiles, d_ppos, d_step_s);
-        else
-            k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, pr, (int)np, nmat, d_tile_id);
-        AOSX_LAUNCHED();
+        multi_update(g0, g1, grp_pair_ptr[g], np);
         if (kstats.enabled) {
             kstats.schur_ms += tm.stop();
             kstats.schur_launches++;
