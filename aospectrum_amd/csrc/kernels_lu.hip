@@ -79,7 +79,6 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     __shared__ float Lre[TILE * LDA], Lim[TILE * LDA];
     __shared__ float Ure[TILE * LDA], Uim[TILE * LDA];
     __shared__ float red[256];
-    __shared__ float piv[2];
     const int m = blockIdx.x, tid = threadIdx.x;
     float *t = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)k * nt + k]);
     float amax = 0.f;
@@ -98,88 +97,135 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     }
     float tau = pivot_rel * sqrtf(red[0]);
     if (!(tau > 0.f)) tau = 1.0e-30f;
+    // Right-looking elimination, one barrier per step: every thread derives
+    // the (possibly replaced) pivot from A(kk,kk) itself, updates its share of
+    // the trailing block with the unscaled multipliers times 1/pivot, and the
+    // scaled column kk is written after the barrier (no later step reads it).
+    // Thread (ti, tj) = (tid / 16, tid % 16) owns rows kk+1+ti+16a and
+    // columns kk+1+tj+16b.
+    const int ti = tid >> 4, tj = tid & 15;
     int local_replaced = 0;
     for (int kk = 0; kk < TILE; ++kk) {
-        if (tid == 0) {
-            float pr = Are[kk * LDA + kk], pi = Aim[kk * LDA + kk];
-            const float mag = sqrtf(pr * pr + pi * pi);
-            if (!(mag >= tau)) {
-                if (mag > 0.f) { pr = pr / mag * tau; pi = pi / mag * tau; }
-                else { pr = tau; pi = 0.f; }
-                ++local_replaced;
-                Are[kk * LDA + kk] = pr;
-                Aim[kk * LDA + kk] = pi;
+        float pr = Are[kk * LDA + kk], pi = Aim[kk * LDA + kk];
+        const float mag = sqrtf(pr * pr + pi * pi);
+        const bool rep = !(mag >= tau);
+        if (rep) {
+            if (mag > 0.f) { pr = pr / mag * tau; pi = pi / mag * tau; }
+            else { pr = tau; pi = 0.f; }
+        }
+        const float d = pr * pr + pi * pi;
+        const float ir = pr / d, ii = -pi / d;
+        float lr[4], li[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int i = kk + 1 + ti + 16 * a;
+            lr[a] = li[a] = 0.f;
+            if (i < TILE) {
+                const float ar = Are[i * LDA + kk], ai = Aim[i * LDA + kk];
+                lr[a] = ar * ir - ai * ii;
+                li[a] = ar * ii + ai * ir;
             }
-            const float d = pr * pr + pi * pi;
-            piv[0] = pr / d;
-            piv[1] = -pi / d;
         }
-        __syncthreads();
-        const float ir = piv[0], ii = piv[1];
-        for (int i = kk + 1 + tid; i < TILE; i += 256) {
-            const float ar = Are[i * LDA + kk], ai = Aim[i * LDA + kk];
-            Are[i * LDA + kk] = ar * ir - ai * ii;
-            Aim[i * LDA + kk] = ar * ii + ai * ir;
-        }
-        __syncthreads();
-        const int w = TILE - 1 - kk;
-        for (int e = tid; e < w * w; e += 256) {
-            const int i = kk + 1 + e / w, j = kk + 1 + e % w;
-            const float lr = Are[i * LDA + kk], li = Aim[i * LDA + kk];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = kk + 1 + tj + 16 * b;
+            if (j >= TILE) break;
             const float ur = Are[kk * LDA + j], ui = Aim[kk * LDA + j];
-            Are[i * LDA + j] -= lr * ur - li * ui;
-            Aim[i * LDA + j] -= lr * ui + li * ur;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = kk + 1 + ti + 16 * a;
+                if (i < TILE) {
+                    Are[i * LDA + j] -= lr[a] * ur - li[a] * ui;
+                    Aim[i * LDA + j] -= lr[a] * ui + li[a] * ur;
+                }
+            }
         }
         __syncthreads();
+        if (tj == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = kk + 1 + ti + 16 * a;
+                if (i < TILE) {
+                    Are[i * LDA + kk] = lr[a];
+                    Aim[i * LDA + kk] = li[a];
+                }
+            }
+        }
+        if (tid == 0 && rep) {
+            Are[kk * LDA + kk] = pr;
+            Aim[kk * LDA + kk] = pi;
+            ++local_replaced;
+        }
     }
+    __syncthreads();
     if (tid == 0 && local_replaced) atomicAdd(replaced + m, local_replaced);
     for (int e = tid; e < TILE2; e += 256) {
         const int r = e / TILE, c = e % TILE;
         t[e] = Are[r * LDA + c];
         t[PLANE + e] = Aim[r * LDA + c];
     }
-    // inv(L): thread c < 64 owns column c; inv(U): thread 64 + r owns row r
-    if (tid < TILE) {
-        const int c = tid;
-        for (int i = 0; i < TILE; ++i) {
-            if (i < c) { Lre[i * LDA + c] = 0.f; Lim[i * LDA + c] = 0.f; continue; }
-            if (i == c) { Lre[i * LDA + c] = 1.f; Lim[i * LDA + c] = 0.f; continue; }
-            float sr = 0.f, si = 0.f;
-            for (int j = c; j < i; ++j) {
+    // inv(L) (unit lower) by columns and inv(U) by rows; four threads share a
+    // column (row) and split each inner product, combined across the four
+    // adjacent lanes:
+    //   X(i,c) = -sum_{j=c}^{i-1} L(i,j) X(j,c),           X(c,c) = 1
+    //   Y(r,j) = (delta_rj - sum_{i=r}^{j-1} Y(r,i) U(i,j)) / U(j,j)
+    const int col = tid >> 2, part = tid & 3;
+    for (int i = 0; i < TILE; ++i) {
+        float sr = 0.f, si = 0.f;
+        if (i > col) {
+            for (int j = col + part; j < i; j += 4) {
                 const float ar = Are[i * LDA + j], ai = Aim[i * LDA + j];
-                const float xr = Lre[j * LDA + c], xi = Lim[j * LDA + c];
+                const float xr = Lre[j * LDA + col], xi = Lim[j * LDA + col];
                 sr += ar * xr - ai * xi;
                 si += ar * xi + ai * xr;
             }
-            Lre[i * LDA + c] = -sr;
-            Lim[i * LDA + c] = -si;
         }
-    } else if (tid < 2 * TILE) {
-        const int r = tid - TILE;
-        for (int j = 0; j < TILE; ++j) {
-            if (j < r) { Ure[r * LDA + j] = 0.f; Uim[r * LDA + j] = 0.f; continue; }
-            float sr = (j == r) ? 1.f : 0.f, si = 0.f;
-            for (int i = r; i < j; ++i) {
-                const float yr = Ure[r * LDA + i], yi = Uim[r * LDA + i];
-                const float ur = Are[i * LDA + j], ui = Aim[i * LDA + j];
-                sr -= yr * ur - yi * ui;
-                si -= yr * ui + yi * ur;
-            }
-            const float dr = Are[j * LDA + j], di = Aim[j * LDA + j];
-            const float dd = dr * dr + di * di;
-            // y_j = s / d
-            Ure[r * LDA + j] = (sr * dr + si * di) / dd;
-            Uim[r * LDA + j] = (si * dr - sr * di) / dd;
+        sr += __shfl_xor(sr, 1);
+        si += __shfl_xor(si, 1);
+        sr += __shfl_xor(sr, 2);
+        si += __shfl_xor(si, 2);
+        if (part == 0) {
+            Lre[i * LDA + col] = i < col ? 0.f : (i == col ? 1.f : -sr);
+            Lim[i * LDA + col] = i <= col ? 0.f : -si;
         }
+        __syncthreads();
     }
-    __syncthreads();
-    float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
+    const int row = col;
+    for (int j = 0; j < TILE; ++j) {
+        float sr = 0.f, si = 0.f;
+        if (j > row) {
+            for (int i = row + part; i < j; i += 4) {
+                const float yr = Ure[row * LDA + i], yi = Uim[row * LDA + i];
+                const float ur = Are[i * LDA + j], ui = Aim[i * LDA + j];
+                sr += yr * ur - yi * ui;
+                si += yr * ui + yi * ur;
+            }
+        }
+        sr += __shfl_xor(sr, 1);
+        si += __shfl_xor(si, 1);
+        sr += __shfl_xor(sr, 2);
+        si += __shfl_xor(si, 2);
+        if (part == 0) {
+            if (j < row) {
+                Ure[row * LDA + j] = 0.f;
+                Uim[row * LDA + j] = 0.f;
+            } else {
+                const float nr = (j == row ? 1.f : 0.f) - sr, ni = -si;
+                const float dr = Are[j * LDA + j], di = Aim[j * LDA + j];
+                const float dd = dr * dr + di * di;
+                Ure[row * LDA + j] = (nr * dr + ni * di) / dd;
+                Uim[row * LDA + j] = (ni * dr - nr * di) / dd;
+            }
+        }
+        __syncthreads();
+    }
+    float *dv = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
     for (int e = tid; e < TILE2; e += 256) {
         const int r = e / TILE, c = e % TILE;
-        d[e] = Lre[r * LDA + c];
-        d[PLANE + e] = Lim[r * LDA + c];
-        d[2 * PLANE + e] = Ure[r * LDA + c];
-        d[3 * PLANE + e] = Uim[r * LDA + c];
+        dv[e] = Lre[r * LDA + c];
+        dv[PLANE + e] = Lim[r * LDA + c];
+        dv[2 * PLANE + e] = Ure[r * LDA + c];
+        dv[3 * PLANE + e] = Uim[r * LDA + c];
     }
 }
 

@@ -444,9 +444,34 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         AOSX_LAUNCHED();
     };
     EventTimer tm(kstats.enabled, stream);
+    // Step groups pay off when a launch covers many matrices; with a few, the
+    // per-step sweep (more, shorter launches) keeps more of the GPU busy.
+    const bool grouped = nmat >= 8;
+    if (!grouped) {
+        for (int k = 0; k < nt; ++k) {
+            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
+            AOSX_LAUNCHED();
+            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+            if (s) {
+                update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                          d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+                AOSX_LAUNCHED();
+            }
+        }
+        for (int k = nt - 1; k >= 0; --k) {
+            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
+            AOSX_LAUNCHED();
+            const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
+            if (s) {
+                update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                          d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
+                AOSX_LAUNCHED();
+            }
+        }
+    }
     // forward: Y_k <- D_k Y_k, then the rows of the group below k, then the
     // group's trailing rows in one launch
-    for (int g = 0; g < ng; ++g) {
+    for (int g = 0; grouped && g < ng; ++g) {
         const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
         for (int k = g0; k < g1; ++k) {
             diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
@@ -464,7 +489,7 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         trailing(g0, g1, d_fwd_rows + fwd_row_ptr[g], fwd_row_ptr[g + 1] - fwd_row_ptr[g]);
     }
     // backward: the same groups in reverse, steps descending
-    for (int g = ng - 1; g >= 0; --g) {
+    for (int g = ng - 1; grouped && g >= 0; --g) {
         const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
         for (int k = g1 - 1; k >= g0; --k) {
             diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
