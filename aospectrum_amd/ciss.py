@@ -12,15 +12,16 @@ Settings beyond the reference's (all default to the reference's behaviour):
 ``precision`` (``fp32`` | ``fp32_ir`` | ``bf16x3_ir`` | ``bf16_ir``),
 ``refinement_steps`` (iterative refinement per shifted solve),
 ``refinement_passes`` (extra contour-filter passes over the Ritz block),
-``k_batch`` (k points factored together) and ``conjugate_pairs`` (solve the
-mirrored shift with the conjugate-transposed factors).
+``k_batch`` (k points factored together), ``conjugate_pairs`` (solve the
+mirrored shift with the conjugate-transposed factors), ``refinement_policy``
+(``fixed`` | ``on_demand``) and ``residual_target_ev``.
 """
 
 from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from time import perf_counter
 from typing import Any
 
@@ -73,6 +74,12 @@ class CissSettings:
     refinement_passes: int = 0
     k_batch: int = 1
     conjugate_pairs: bool = False
+    # "fixed": every shifted solve takes `refinement_steps` FP64-residual IR
+    # steps.  "on_demand": solve without refinement first and re-solve, with
+    # `refinement_steps`, only the k points whose FP64 eigenpair residual
+    # exceeds `residual_target_ev`.
+    refinement_policy: str = "fixed"
+    residual_target_ev: float = 1.0e-4
 
     def __post_init__(self) -> None:
         if (
@@ -89,6 +96,10 @@ class CissSettings:
             raise InputError("refinement counts must be >= 0 and k_batch >= 1")
         if self.conjugate_pairs and self.integration_points % 2:
             raise InputError("conjugate_pairs needs an even number of integration points")
+        if self.refinement_policy not in ("fixed", "on_demand"):
+            raise InputError("refinement_policy must be 'fixed' or 'on_demand'")
+        if not self.residual_target_ev > 0.0:
+            raise InputError("residual_target_ev must be positive")
 
     @property
     def search_dimension(self) -> int:
@@ -157,7 +168,44 @@ def band_solve(
     """Run the CISS Band path for a batch of k points (on-device assembly) or
     of explicit H/S value arrays on the bundle's pattern.  ``intervals``
     ([nk, 2], absolute eV) gives every k point its own window; otherwise all
-    use ``interval``."""
+    use ``interval``.
+
+    With ``refinement_policy = "on_demand"`` the batch is first solved without
+    iterative refinement; the k points whose FP64 eigenpair residual exceeds
+    ``residual_target_ev`` are solved again with ``refinement_steps`` steps and
+    their results replace the first ones (``info["refined"]`` lists them)."""
+    if settings.refinement_policy == "fixed" or settings.refinement_steps == 0:
+        results, info = _band_call(native, settings, interval, kpoints, h_data, s_data, intervals)
+        info["refined"] = []
+        return results, info
+    first = replace(settings, refinement_steps=0, refinement_policy="fixed")
+    results, info = _band_call(native, first, interval, kpoints, h_data, s_data, intervals)
+    redo = [i for i, r in enumerate(results) if r.maximum_residual_ev > settings.residual_target_ev]
+    info["refined"] = redo
+    if redo:
+        fixed = replace(settings, refinement_policy="fixed")
+        sel = np.asarray(redo)
+        again, info2 = _band_call(
+            native,
+            fixed,
+            interval,
+            None if kpoints is None else np.asarray(kpoints, dtype=np.float64).reshape(-1, 3)[sel],
+            None if h_data is None else np.asarray(h_data).reshape(len(results), -1)[sel],
+            None if s_data is None else np.asarray(s_data).reshape(len(results), -1)[sel],
+            None if intervals is None else np.asarray(intervals, dtype=np.float64).reshape(-1, 2)[sel],
+        )
+        for i, r in zip(redo, again):
+            results[i] = r
+        info["factorizations"] += info2["factorizations"]
+        info["replaced_pivots"] += info2["replaced_pivots"]
+        info["wall_seconds"] += info2["wall_seconds"]
+        info["peak_device_bytes"] = max(info["peak_device_bytes"], info2["peak_device_bytes"])
+        for name, v in info2["stage_seconds"].items():
+            info["stage_seconds"][name] += v
+    return results, info
+
+
+def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
     lib = N.load()
     if (kpoints is None) == (h_data is None):
         raise InputError("give either k points or explicit H/S values")

@@ -12,7 +12,7 @@ k points over one spawned process per GPU, as the reference does
 
 Extra ``[solver]`` keys (defaults keep the reference's behaviour):
 ``precision``, ``refinement_steps``, ``refinement_passes``, ``k_batch``,
-``conjugate_pairs``.
+``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``.
 """
 
 from __future__ import annotations
@@ -133,6 +133,8 @@ def read_band_config(path: str | Path) -> BandConfig:
             refinement_passes=int(solver.get("refinement_passes", 0)),
             k_batch=int(solver.get("k_batch", 1)),
             conjugate_pairs=conj,
+            refinement_policy=str(solver.get("refinement_policy", "fixed")),
+            residual_target_ev=float(solver.get("residual_target_ev", 1.0e-4)),
         )
     except (TypeError, ValueError) as exc:
         raise InputError("invalid Band solver setting") from exc

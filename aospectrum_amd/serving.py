@@ -105,6 +105,8 @@ def _settings_key(s: CissSettings) -> tuple:
         s.refinement_steps,
         s.refinement_passes,
         s.conjugate_pairs,
+        s.refinement_policy,
+        s.residual_target_ev,
     )
 
 

@@ -133,6 +133,54 @@ def test_band_solve_matches_fp64_oracle(small2d, small2d_pattern, native_small, 
             assert r.maximum_residual_ev <= 1e-4
 
 
+MIXED_ON_DEMAND = CissSettings(
+    contour_batch_size=16,
+    k_batch=2,
+    conjugate_pairs=True,
+    precision="bf16x3_ir",
+    refinement_steps=2,
+    refinement_passes=2,
+    refinement_policy="on_demand",
+)
+
+
+def test_on_demand_refinement_matches_oracle(small2d, small2d_pattern, native_small):
+    """Refinement on demand matches the FP64 oracle (whichever k points it
+    had to re-solve)."""
+    kpts = np.array([[0.13, 0.37, 0.0], [0.0, 0.0, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    say("band solve on_demand")
+    results, info = band_solve(native_small, MIXED_ON_DEMAND, interval, kpoints=kpts)
+    say("  refined", info["refined"])
+    for k, r in zip(kpts, results):
+        ref = _oracle_window(small2d, small2d_pattern, k, interval.lower_ev, interval.upper_ev)
+        say(f"  k={k} count={r.energies_ev.size} ref={ref.size} res={r.maximum_residual_ev:.2e}")
+        assert r.energies_ev.size == ref.size
+        assert np.max(np.abs(np.sort(r.energies_ev) - ref)) < 1e-4
+        assert r.maximum_residual_ev <= 1e-4
+
+
+def test_on_demand_refinement_resolves_missed_targets(native_small):
+    """A residual target no solve can meet sends every k point through the
+    refined re-solve, whose results are those of the fixed policy."""
+    from dataclasses import replace
+
+    kpts = np.array([[0.13, 0.37, 0.0], [0.25, 0.0, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    strict = replace(MIXED_ON_DEMAND, residual_target_ev=1e-12)
+    fixed = replace(MIXED_ON_DEMAND, refinement_policy="fixed")
+    say("band solve on_demand, unreachable target")
+    got, info = band_solve(native_small, strict, interval, kpoints=kpts)
+    want, info_fixed = band_solve(native_small, fixed, interval, kpoints=kpts)
+    assert info["refined"] == [0, 1]
+    # the first (unrefined) pass plus the refined re-solve
+    assert info["factorizations"] == 2 * info_fixed["factorizations"]
+    for a, b in zip(got, want):
+        assert a.energies_ev.size == b.energies_ev.size
+        assert np.max(np.abs(a.energies_ev - b.energies_ev), initial=0.0) <= 1e-6
+        assert a.maximum_residual_ev == pytest.approx(b.maximum_residual_ev, rel=1e-6)
+
+
 def test_band_solve_matches_oracle_ciss(small2d, small2d_pattern, native_small):
     """Single-pass plain CISS: same energies and rank as the CPU restatement
     of the reference algorithm (same probes, same settings)."""
