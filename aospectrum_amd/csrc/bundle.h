@@ -92,6 +92,9 @@ struct Bundle {
     std::vector<int64_t> grp_pair_ptr;
     std::vector<double> grp_updates;  // tile updates of each trailing launch
     int32_t *d_pairs = nullptr;
+    // the same trailing targets as 2 x 2 blocks (i0, i1, j0, j1; -1 absent)
+    std::vector<int64_t> grp_quad_ptr;
+    int32_t *d_quads = nullptr;
     // BF16 precisions: the panels of a group also write split copies of their
     // tiles; L(i,t) sits in slot ppos[t*nt+i] of the group's split area and
     // U(t,j) in slot ppos[t*nt+j] + step_s[t] (split_tiles slots per matrix).

@@ -600,4 +600,175 @@ template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, 
                                                     const int32_t *, const uint16_t *, int64_t, const int32_t *,
                                                     const int32_t *);
 
+// Trailing update of a step group on 2 x 2 blocks of targets (BF16 precisions):
+// quad q = (i0, i1, j0, j1) (an index -1 is absent); wave w updates the full
+// 64 x 64 target (i_{w/2}, j_{w%2}) over the steps where both of its operands
+// exist, from the split tiles L(i0,t), L(i1,t), U(t,j0), U(t,j1) staged once
+// per step for all four waves -- half the operand traffic per target update
+// of the one-target form.  One block per CU (147 KiB of LDS); the next step's
+// four tiles are loaded into registers while the current products run.
+__device__ __forceinline__ void mma_split_quad_chunk(const uint16_t *As, const uint16_t *Bs, floatx16 (&cre)[4],
+                                                     floatx16 (&cim)[4], int lane, int k0, bool three) {
+    const int kb = (lane >> 5) * 8, r = lane & 31;
+    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ao = (h * 32 + r) * SLD + k0 + kb;
+        prh[h] = lds8(As + ao);
+        pih[h] = lds8(As + 2 * SPL + ao);
+        urh[h] = lds8(Bs + ao);
+        uih[h] = lds8(Bs + 2 * SPL + ao);
+        if (three) {
+            prl[h] = lds8(As + SPL + ao);
+            pil[h] = lds8(As + 3 * SPL + ao);
+            url[h] = lds8(Bs + SPL + ao);
+            uil[h] = lds8(Bs + 3 * SPL + ao);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const bf16x8 npih = neg8(pih[a]);
+        const bf16x8 npil = three ? neg8(pil[a]) : npih;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            floatx16 &cr = cre[2 * a + b], &ci = cim[2 * a + b];
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], urh[b], cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uih[b], cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], uih[b], ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih[a], urh[b], ci, 0, 0, 0);
+            if (three) {
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], url[b], cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl[a], urh[b], cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uil[b], cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npil, uih[b], cr, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], uil[b], ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl[a], uih[b], ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih[a], url[b], ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pil[a], urh[b], ci, 0, 0, 0);
+            }
+        }
+    }
+}
+
+// Accumulators of a full 64 x 64 tile held by one wave: block 2a+b covers rows
+// 32a.., columns 32b.. with the 32 x 32 MFMA layout.
+__device__ __forceinline__ void load_acc_tile(const float *__restrict__ t, floatx16 (&cre)[4], floatx16 (&cim)[4],
+                                              int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r0 = (q >> 1) * 32 + 4 * (lane >> 5), c = (q & 1) * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = r0 + 8 * (r >> 2) + (r & 3);
+            cre[q][r] = t[row * TILE + c];
+            cim[q][r] = t[PLANE + row * TILE + c];
+        }
+    }
+}
+__device__ __forceinline__ void store_acc_tile(float *__restrict__ t, const floatx16 (&cre)[4],
+                                               const floatx16 (&cim)[4], int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r0 = (q >> 1) * 32 + 4 * (lane >> 5), c = (q & 1) * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = r0 + 8 * (r >> 2) + (r & 3);
+            t[row * TILE + c] = cre[q][r];
+            t[PLANE + row * TILE + c] = cim[q][r];
+        }
+    }
+}
+
+template <bool THREE>
+__global__ void __launch_bounds__(256, 1) k_schur_quad_split(
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ quads,
+    int nquads, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
+    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    __shared__ __attribute__((aligned(16))) uint16_t A0[4 * SPL], A1[4 * SPL], B0[4 * SPL], B1[4 * SPL];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = (int)(item / nquads), q = (int)(item % nquads);
+    const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
+    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
+    // step masks: rows (L present), columns (U present), this wave's target
+    uint32_t rm[2] = {0u, 0u}, cm[2] = {0u, 0u};
+    for (int t = g0; t < g1; ++t) {
+        const int32_t *pp = ppos + (int64_t)t * nt;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (I[h] >= 0 && pp[I[h]] >= 0) rm[h] |= 1u << (t - g0);
+            if (J[h] >= 0 && pp[J[h]] >= 0) cm[h] |= 1u << (t - g0);
+        }
+    }
+    const int ia = wave >> 1, jb = wave & 1;
+    uint32_t mine = rm[ia] & cm[jb];
+    // steps where some wave has work, and the tiles they need
+    uint32_t all = (rm[0] & cm[0]) | (rm[0] & cm[1]) | (rm[1] & cm[0]) | (rm[1] & cm[1]);
+    const uint32_t need_r[2] = {rm[0] & (cm[0] | cm[1]), rm[1] & (cm[0] | cm[1])};
+    const uint32_t need_c[2] = {cm[0] & (rm[0] | rm[1]), cm[1] & (rm[0] | rm[1])};
+    float *C = mine ? tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)I[ia] * nt + J[jb]]) : nullptr;
+    floatx16 cre[4], cim[4];
+    if (mine) load_acc_tile(C, cre, cim, lane);
+    if (all) {
+        // Offsets of the four split tiles of step t; an absent tile points at
+        // a present one, so that all four loads are issued without branches
+        // (they are not staged).
+        auto offsets = [&](int t, int64_t (&o)[4]) {
+            const uint32_t bit = 1u << (t - g0);
+            const int32_t *pp = ppos + (int64_t)t * nt;
+            const int64_t s0 = (need_r[0] & bit) ? pp[I[0]] : -1, s1 = (need_r[1] & bit) ? pp[I[1]] : -1;
+            const int64_t s2 = (need_c[0] & bit) ? pp[J[0]] + step_s[t] : -1;
+            const int64_t s3 = (need_c[1] & bit) ? pp[J[1]] + step_s[t] : -1;
+            const int64_t any = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : (s2 >= 0 ? s2 : s3));
+            o[0] = (s0 >= 0 ? s0 : any) * SPLIT_TILE;
+            o[1] = (s1 >= 0 ? s1 : any) * SPLIT_TILE;
+            o[2] = (s2 >= 0 ? s2 : any) * SPLIT_TILE;
+            o[3] = (s3 >= 0 ? s3 : any) * SPLIT_TILE;
+        };
+        int t = g0 + __builtin_ctz(all);
+        all &= all - 1;
+        int64_t o[4];
+        offsets(t, o);
+        u32x32 ra0 = fetch_split(sp + o[0], tid), ra1 = fetch_split(sp + o[1], tid);
+        u32x32 rb0 = fetch_split(sp + o[2], tid), rb1 = fetch_split(sp + o[3], tid);
+        for (;;) {
+            const uint32_t bit = 1u << (t - g0);
+            __syncthreads();  // the previous step's products are done with LDS
+            if (need_r[0] & bit) put_split(A0, ra0, tid);
+            if (need_r[1] & bit) put_split(A1, ra1, tid);
+            if (need_c[0] & bit) put_split(B0, rb0, tid);
+            if (need_c[1] & bit) put_split(B1, rb1, tid);
+            __syncthreads();
+            const bool more = all != 0;
+            const int tcur = t;
+            // the next step's tiles (the current step's again after the last)
+            if (more) {
+                t = g0 + __builtin_ctz(all);
+                all &= all - 1;
+            }
+            offsets(t, o);
+            ra0 = fetch_split(sp + o[0], tid);
+            ra1 = fetch_split(sp + o[1], tid);
+            rb0 = fetch_split(sp + o[2], tid);
+            rb1 = fetch_split(sp + o[3], tid);
+            if (mine & (1u << (tcur - g0))) {
+                const uint16_t *As = ia ? A1 : A0, *Bs = jb ? B1 : B0;
+#pragma unroll
+                for (int k0 = 0; k0 < TILE; k0 += 16) mma_split_quad_chunk(As, Bs, cre, cim, lane, k0, THREE);
+            }
+            if (!more) break;
+        }
+    }
+    if (mine) store_acc_tile(C, cre, cim, lane);
+}
+
+template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                  const int32_t *);
+template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                                   const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                   const int32_t *);
+
 }  // namespace aosx

@@ -70,7 +70,7 @@ Bundle::~Bundle() {
         for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
                         (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
-                        (void *)d_atom_off, (void *)d_pairs, (void *)d_ppos, (void *)d_step_s,
+                        (void *)d_atom_off, (void *)d_pairs, (void *)d_quads, (void *)d_ppos, (void *)d_step_s,
                         (void *)d_fwd_rows, (void *)d_bwd_rows})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -186,7 +186,8 @@ void Bundle::plan_schur(int group) {
     grp_begin.clear();
     grp_pair_ptr.assign(1, 0);
     grp_updates.clear();
-    std::vector<int32_t> pairs;
+    std::vector<int32_t> pairs, quads;
+    grp_quad_ptr.assign(1, 0);
     std::vector<uint8_t> mark((size_t)nt * nt, 0);
     std::vector<int32_t> ppos((size_t)nt * nt, -1), step_s(nt, 0);
     split_tiles = 0;
@@ -213,17 +214,31 @@ void Bundle::plan_schur(int group) {
         std::sort(rows.begin(), rows.end());
         rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
         const int r = (int)rows.size();
+        auto marked = [&](int a, int b) { return a < r && b < r && mark[(size_t)rows[a] * nt + rows[b]] != 0; };
         for (int bi = 0; bi < r; bi += 8)
             for (int bj = 0; bj < r; bj += 8)
                 for (int a = bi; a < std::min(r, bi + 8); ++a)
-                    for (int b = bj; b < std::min(r, bj + 8); ++b) {
-                        uint8_t &f = mark[(size_t)rows[a] * nt + rows[b]];
-                        if (!f) continue;
-                        f = 0;
-                        pairs.push_back(rows[a]);
-                        pairs.push_back(rows[b]);
-                    }
+                    for (int b = bj; b < std::min(r, bj + 8); ++b)
+                        if (marked(a, b)) {
+                            pairs.push_back(rows[a]);
+                            pairs.push_back(rows[b]);
+                        }
+        // quads: 2 x 2 blocks of consecutive rows and columns, in 4 x 4 blocks
+        // of quads; -1 pads an odd row count
+        for (int bi = 0; bi < r; bi += 8)
+            for (int bj = 0; bj < r; bj += 8)
+                for (int a = bi; a < std::min(r, bi + 8); a += 2)
+                    for (int b = bj; b < std::min(r, bj + 8); b += 2)
+                        if (marked(a, b) || marked(a, b + 1) || marked(a + 1, b) || marked(a + 1, b + 1)) {
+                            quads.push_back(rows[a]);
+                            quads.push_back(a + 1 < r ? rows[a + 1] : -1);
+                            quads.push_back(rows[b]);
+                            quads.push_back(b + 1 < r ? rows[b + 1] : -1);
+                        }
+        for (int a = 0; a < r; ++a)
+            for (int b = 0; b < r; ++b) mark[(size_t)rows[a] * nt + rows[b]] = 0;
         grp_pair_ptr.push_back((int64_t)pairs.size() / 2);
+        grp_quad_ptr.push_back((int64_t)quads.size() / 4);
         grp_updates.push_back(updates);
         split_tiles = std::max(split_tiles, slot);
     }
@@ -277,8 +292,9 @@ void Bundle::plan_schur(int group) {
         fwd_row_ptr.push_back((int64_t)frows.size());
         bwd_row_ptr.push_back((int64_t)brows.size());
     }
-    for (int32_t *p : {d_pairs, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows})
+    for (int32_t *p : {d_pairs, d_quads, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows})
         if (p) AOSX_HIP(hipFree(p));
+    d_quads = upload_vec(quads, resident_bytes);
     d_fwd_rows = upload_vec(frows, resident_bytes);
     d_bwd_rows = upload_vec(brows, resident_bytes);
     d_pairs = upload_vec(pairs, resident_bytes);
@@ -413,7 +429,20 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
         const int64_t np = grp_pair_ptr[g + 1] - grp_pair_ptr[g];
         if (np == 0) continue;
         EventTimer tm(kstats.enabled, stream);
-        multi_update(g0, g1, grp_pair_ptr[g], np);
+        if (split_mode) {
+            const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
+            const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
+            const int32_t *qd = d_quads + 4 * grp_quad_ptr[g];
+            if (precision == AOSX_PRECISION_BF16X3_IR)
+                k_schur_quad_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat,
+                                                                     d_tile_id, split, split_tiles, d_ppos, d_step_s);
+            else
+                k_schur_quad_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat,
+                                                                      d_tile_id, split, split_tiles, d_ppos, d_step_s);
+            AOSX_LAUNCHED();
+        } else {
+            multi_update(g0, g1, grp_pair_ptr[g], np);
+        }
         if (kstats.enabled) {
             kstats.schur_ms += tm.stop();
             kstats.schur_launches++;
