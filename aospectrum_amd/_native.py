@@ -232,6 +232,35 @@ class KernelStats(C.Structure):
     ]
 
 
+AUTOTUNE_MAX_CANDIDATES = 16
+
+
+class AutotuneRequest(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32),
+        ("struct_size", C.c_uint32),
+        ("matrices", C.c_int32),
+        ("precision", C.c_int32),
+        ("n_candidates", C.c_int32),
+        ("repeats", C.c_int32),
+        ("candidates", _I32P),
+    ]
+
+
+class AutotuneResult(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32),
+        ("struct_size", C.c_uint32),
+        ("status", C.c_int32),
+        ("hip_status", C.c_int32),
+        ("chosen_group", C.c_int32),
+        ("n_candidates", C.c_int32),
+        ("candidates", C.c_int32 * AUTOTUNE_MAX_CANDIDATES),
+        ("factor_ms", C.c_double * AUTOTUNE_MAX_CANDIDATES),
+        ("error_message", C.c_char * ERROR_CAPACITY),
+    ]
+
+
 _STRUCTS = (
     BundleRequest,
     BundleResult,
@@ -242,6 +271,8 @@ _STRUCTS = (
     BandRequest,
     BandResult,
     KernelStats,
+    AutotuneRequest,
+    AutotuneResult,
 )
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "csrc" / "libaosx_hip.so"
@@ -283,6 +314,7 @@ def load() -> C.CDLL:
         "aosx_band_solve_v1": [_P, C.POINTER(BandRequest), C.POINTER(BandResult)],
         "aosx_kernel_stats_read_v1": [_P, C.c_int32, C.POINTER(KernelStats)],
         "aosx_bundle_trim_v1": [_P, C.POINTER(C.c_uint64)],
+        "aosx_bundle_autotune_v1": [_P, C.POINTER(AutotuneRequest), C.POINTER(AutotuneResult)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -302,6 +334,7 @@ EXPORTED_SYMBOLS = (
     "aosx_kernel_stats_read_v1",
     "aosx_struct_sizes_v1",
     "aosx_bundle_trim_v1",
+    "aosx_bundle_autotune_v1",
 )
 
 

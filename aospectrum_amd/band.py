@@ -188,6 +188,7 @@ class BandCalculation:
     pattern: BlochPattern = field(init=False)
     workspace: Any = field(init=False)
     warnings: tuple[str, ...] = field(init=False)
+    autotune_result: dict | None = field(init=False)
 
     def __post_init__(self) -> None:
         self.solve_interval_ev = self.display_interval_ev.expanded(CONTINUITY_GUARD_EV)
@@ -197,6 +198,11 @@ class BandCalculation:
         )
         self.pattern = BlochPattern.from_bundle(self.bundle, device=self.device)
         self.workspace = self.pattern.workspace(complex_values=True)
+        self.autotune_result = None
+        if self.settings.autotune and self.solver is None:
+            self.autotune_result = self.pattern.native.autotune(
+                self.settings.factorizations_per_launch, self.settings.precision
+            )
         warnings = []
         if self.bundle.basis.spinor_width != 1:
             warnings.append("spinor input is outside the current non-SOC Band scope; the result may be inaccurate")

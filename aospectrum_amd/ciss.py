@@ -14,7 +14,7 @@ Settings beyond the reference's (all default to the reference's behaviour):
 ``refinement_passes`` (extra contour-filter passes over the Ritz block),
 ``k_batch`` (k points factored together), ``conjugate_pairs`` (solve the
 mirrored shift with the conjugate-transposed factors), ``refinement_policy``
-(``fixed`` | ``on_demand``) and ``residual_target_ev``.
+(``fixed`` | ``on_demand``), ``residual_target_ev`` and ``autotune``.
 """
 
 from __future__ import annotations
@@ -80,6 +80,14 @@ class CissSettings:
     # exceeds `residual_target_ev`.
     refinement_policy: str = "fixed"
     residual_target_ev: float = 1.0e-4
+    # time the step-group sizes of the tile LU on the bundle before the first
+    # solve and keep the fastest (NativeBundle.autotune)
+    autotune: bool = False
+
+    @property
+    def factorizations_per_launch(self) -> int:
+        per_k = self.integration_points // 2 if self.conjugate_pairs else self.contour_batch_size
+        return self.k_batch * per_k
 
     def __post_init__(self) -> None:
         if (

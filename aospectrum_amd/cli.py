@@ -12,7 +12,7 @@ k points over one spawned process per GPU, as the reference does
 
 Extra ``[solver]`` keys (defaults keep the reference's behaviour):
 ``precision``, ``refinement_steps``, ``refinement_passes``, ``k_batch``,
-``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``.
+``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``, ``autotune``.
 """
 
 from __future__ import annotations
@@ -121,8 +121,9 @@ def read_band_config(path: str | Path) -> BandConfig:
     try:
         ref_value = None if ref_value is None else float(ref_value)
         conj = solver.get("conjugate_pairs", False)
-        if not isinstance(conj, bool):
-            raise TypeError("conjugate_pairs must be a boolean")
+        tune = solver.get("autotune", False)
+        if not isinstance(conj, bool) or not isinstance(tune, bool):
+            raise TypeError("conjugate_pairs and autotune must be booleans")
         settings = CissSettings(
             integration_points=int(solver.get("integration_points", 32)),
             contour_batch_size=int(solver.get("contour_batch_size", resources.get("contour_batch_size", 1))),
@@ -135,6 +136,7 @@ def read_band_config(path: str | Path) -> BandConfig:
             conjugate_pairs=conj,
             refinement_policy=str(solver.get("refinement_policy", "fixed")),
             residual_target_ev=float(solver.get("residual_target_ev", 1.0e-4)),
+            autotune=tune,
         )
     except (TypeError, ValueError) as exc:
         raise InputError("invalid Band solver setting") from exc

@@ -412,12 +412,44 @@ int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_s
     return 0;
 }
 
+int aosx_bundle_autotune_v1(aosx_bundle *bundle, const aosx_autotune_request_v1 *q, aosx_autotune_result_v1 *res) {
+    if (!q || !res) return 1;
+    begin(res);
+    if (!abi_ok(q)) return fail(res, AOSX_ABI_MISMATCH, 0, "request ABI mismatch");
+    if (!bundle) return fail(res, AOSX_NULL_POINTER, 0, "bundle is null");
+    Bundle &b = *bundle->impl;
+    if (b.device < 0) return fail(res, AOSX_NO_DEVICE, 0, "bundle was created without a device");
+    if (q->matrices < 1 || q->matrices > 256) return fail(res, AOSX_BATCH_DIMENSIONS, 0, "matrices must be 1..256");
+    if (q->precision < 0 || q->precision > 3 || q->repeats < 1 || q->repeats > 100)
+        return fail(res, AOSX_INVALID_SETTINGS, 0, "invalid precision or repeats");
+    if (q->n_candidates < 0 || q->n_candidates > AOSX_AUTOTUNE_MAX_CANDIDATES || (q->n_candidates && !q->candidates))
+        return fail(res, AOSX_INVALID_SETTINGS, 0, "invalid candidate list");
+    std::vector<int> cands;
+    for (int i = 0; i < q->n_candidates; ++i) {
+        if (q->candidates[i] < 1 || q->candidates[i] > 32)
+            return fail(res, AOSX_INVALID_SETTINGS, 0, "group sizes must be 1..32");
+        cands.push_back(q->candidates[i]);
+    }
+    if (cands.empty()) cands = {4, 8, 12, 16, 24, 32};
+    return guarded(res, [&]() -> int {
+        std::vector<double> ms;
+        res->chosen_group = b.autotune(q->matrices, q->precision, cands, q->repeats, ms);
+        res->n_candidates = (int32_t)cands.size();
+        for (size_t i = 0; i < cands.size(); ++i) {
+            res->candidates[i] = cands[i];
+            res->factor_ms[i] = ms[i];
+        }
+        return 0;
+    });
+}
+
 int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity) {
     const int64_t all[] = {(int64_t)sizeof(aosx_bundle_request_v1), (int64_t)sizeof(aosx_bundle_result_v1),
                            (int64_t)sizeof(aosx_status_v1),         (int64_t)sizeof(aosx_assemble_request_v1),
                            (int64_t)sizeof(aosx_shift_request_v1),  (int64_t)sizeof(aosx_shift_result_v1),
                            (int64_t)sizeof(aosx_band_request_v1),   (int64_t)sizeof(aosx_band_result_v1),
-                           (int64_t)sizeof(aosx_kernel_stats_v1)};
+                           (int64_t)sizeof(aosx_kernel_stats_v1),   (int64_t)sizeof(aosx_autotune_request_v1),
+                           (int64_t)sizeof(aosx_autotune_result_v1)};
     const int count = (int)(sizeof(all) / sizeof(all[0]));
     for (int i = 0; i < count && i < capacity; ++i) sizes[i] = all[i];
     return count;

@@ -50,6 +50,24 @@ struct KernelStats {
     double assemble_bytes = 0.0;
 };
 
+// Categories of device-timed launch ranges (aosx_kernel_stats_v1 fields).
+enum RangeKind { RANGE_SCHUR = 0, RANGE_INNER, RANGE_PANEL, RANGE_GETRF, RANGE_SOLVE };
+
+// A pair of events around launches on the bundle's stream, with the
+// algorithmic work of what they bracket (flops, or bytes for solves).
+struct TimedRange {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0;
+    double work = 0.0;
+};
+
+// A captured launch sequence: the executable graph and the timed ranges whose
+// event-record nodes it contains (read after every replay).
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    std::vector<TimedRange> ranges;
+};
+
 struct CissParams {
     int64_t nk = 0;
     double lower = 0.0, upper = 0.0;
@@ -114,6 +132,19 @@ struct Bundle {
     uint64_t resident_bytes = 0;
     uint64_t peak_bytes = 0;
     KernelStats kstats;
+    // Launch sequences of factor() and solve() run as HIP graphs, captured on
+    // the first call with a given key (sizes, precision, buffer addresses)
+    // and replayed afterwards; AOSX_GRAPHS=0 runs them eagerly.
+    bool use_graphs = true;
+    std::map<std::vector<int64_t>, GraphEntry> graphs;
+    std::vector<TimedRange> *ranges = nullptr;  // collecting ranges of the running sequence
+    bool capturing = false;
+    int range_begin(int kind);
+    void range_end(int id, double work);
+    void flush_ranges(std::vector<TimedRange> &rs, bool destroy);
+    void clear_graphs();
+    template <typename F>
+    void run_sequence(const std::vector<int64_t> &key, F &&body);
     struct Slot {
         void *ptr = nullptr;
         size_t bytes = 0;
@@ -127,6 +158,11 @@ struct Bundle {
         return static_cast<T *>(raw(name, count * sizeof(T)));
     }
     void plan_schur(int group);
+    // times `repeats` factorisations of nmat shifted copies of H(Gamma), S(Gamma)
+    // per candidate group size, keeps the fastest and returns it
+    int autotune(int nmat, int precision, const std::vector<int> &candidates, int repeats,
+                 std::vector<double> &factor_ms);
+    uint64_t plan_bytes = 0;  // device bytes of the step-group plan
     void release_workspace();
     uint64_t workspace_bytes() const;
     size_t factor_floats() const { return (size_t)ts.n_tiles * 2 * 64 * 64; }

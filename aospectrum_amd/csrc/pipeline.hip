@@ -63,9 +63,116 @@ struct EventTimer {
 };
 }  // namespace
 
+// ---------------------------------------------------- timed ranges, graphs
+int Bundle::range_begin(int kind) {
+    if (!kstats.enabled || ranges == nullptr) return -1;
+    TimedRange r;
+    r.kind = kind;
+    AOSX_HIP(hipEventCreate(&r.a));
+    AOSX_HIP(hipEventCreate(&r.b));
+    AOSX_HIP(hipEventRecordWithFlags(r.a, stream, capturing ? hipEventRecordExternal : 0));
+    ranges->push_back(r);
+    return (int)ranges->size() - 1;
+}
+
+void Bundle::range_end(int id, double work) {
+    if (id < 0) return;
+    TimedRange &r = (*ranges)[id];
+    AOSX_HIP(hipEventRecordWithFlags(r.b, stream, capturing ? hipEventRecordExternal : 0));
+    r.work = work;
+}
+
+// Waits for the last range, adds every range to the kernel statistics.
+void Bundle::flush_ranges(std::vector<TimedRange> &rs, bool destroy) {
+    if (rs.empty()) return;
+    AOSX_HIP(hipEventSynchronize(rs.back().b));
+    for (TimedRange &r : rs) {
+        float ms = 0.f;
+        AOSX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        switch (r.kind) {
+        case RANGE_SCHUR: kstats.schur_ms += ms; kstats.schur_launches++; kstats.schur_flops += r.work; break;
+        case RANGE_INNER: kstats.inner_ms += ms; kstats.inner_launches++; kstats.inner_flops += r.work; break;
+        case RANGE_PANEL: kstats.panel_ms += ms; kstats.panel_launches++; break;
+        case RANGE_GETRF: kstats.getrf_ms += ms; kstats.getrf_launches++; break;
+        case RANGE_SOLVE: kstats.solve_ms += ms; kstats.solve_launches++; kstats.solve_bytes += r.work; break;
+        }
+        if (destroy) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    if (destroy) rs.clear();
+}
+
+void Bundle::clear_graphs() {
+    for (auto &kv : graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        for (TimedRange &r : kv.second.ranges) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    graphs.clear();
+}
+
+// Runs `body` -- kernel launches and event records on `stream` only, no
+// allocation, copy or synchronisation -- through the graph cached under
+// `key`, capturing it on first use.
+template <typename F>
+void Bundle::run_sequence(const std::vector<int64_t> &key_in, F &&body) {
+    std::vector<int64_t> key = key_in;
+    key.push_back(kstats.enabled ? 1 : 0);
+    if (!use_graphs) {
+        std::vector<TimedRange> rs;
+        ranges = &rs;
+        try {
+            body();
+        } catch (...) {
+            ranges = nullptr;
+            flush_ranges(rs, true);
+            throw;
+        }
+        ranges = nullptr;
+        flush_ranges(rs, true);
+        return;
+    }
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+        if (graphs.size() >= 512) clear_graphs();
+        GraphEntry e;
+        ranges = &e.ranges;
+        capturing = true;
+        AOSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            capturing = false;
+            ranges = nullptr;
+            for (TimedRange &r : e.ranges) {
+                (void)hipEventDestroy(r.a);
+                (void)hipEventDestroy(r.b);
+            }
+            throw;
+        }
+        capturing = false;
+        ranges = nullptr;
+        AOSX_HIP(hipStreamEndCapture(stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        AOSX_HIP(ie);
+        it = graphs.emplace(key, std::move(e)).first;
+    }
+    AOSX_HIP(hipGraphLaunch(it->second.exec, stream));
+    if (!it->second.ranges.empty()) flush_ranges(it->second.ranges, false);
+}
+
 Bundle::~Bundle() {
     if (device >= 0) {
         (void)hipSetDevice(device);
+        clear_graphs();
         release_workspace();
         for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
@@ -104,6 +211,7 @@ void *Bundle::raw(const std::string &name, size_t bytes) {
 }
 
 void Bundle::release_workspace() {
+    clear_graphs();
     for (auto &kv : ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     ws.clear();
@@ -173,6 +281,7 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     int group = 16;
     if (const char *env = std::getenv("AOSX_SCHUR_GROUP")) group = std::min(32, std::max(1, std::atoi(env)));
     plan_schur(group);
+    if (const char *env = std::getenv("AOSX_GRAPHS")) use_graphs = std::atoi(env) != 0;
     peak_bytes = resident_bytes;
 }
 
@@ -292,14 +401,61 @@ void Bundle::plan_schur(int group) {
         fwd_row_ptr.push_back((int64_t)frows.size());
         bwd_row_ptr.push_back((int64_t)brows.size());
     }
+    // the launch sequences captured so far follow the old plan
+    clear_graphs();
     for (int32_t *p : {d_pairs, d_quads, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows})
         if (p) AOSX_HIP(hipFree(p));
-    d_quads = upload_vec(quads, resident_bytes);
-    d_fwd_rows = upload_vec(frows, resident_bytes);
-    d_bwd_rows = upload_vec(brows, resident_bytes);
-    d_pairs = upload_vec(pairs, resident_bytes);
-    d_ppos = upload_vec(ppos, resident_bytes);
-    d_step_s = upload_vec(step_s, resident_bytes);
+    uint64_t pb = 0;
+    d_quads = upload_vec(quads, pb);
+    d_fwd_rows = upload_vec(frows, pb);
+    d_bwd_rows = upload_vec(brows, pb);
+    d_pairs = upload_vec(pairs, pb);
+    d_ppos = upload_vec(ppos, pb);
+    d_step_s = upload_vec(step_s, pb);
+    resident_bytes = resident_bytes - plan_bytes + pb;
+    plan_bytes = pb;
+}
+
+int Bundle::autotune(int nmat, int precision, const std::vector<int> &candidates, int repeats,
+                     std::vector<double> &factor_ms) {
+    AOSX_HIP(hipSetDevice(device));
+    const double gamma[3] = {0.0, 0.0, 0.0};
+    float2 *d_h = buf<float2>("H", (size_t)nnz);
+    float2 *d_s = buf<float2>("S", (size_t)nnz);
+    struct StatsOff {  // no kernel statistics while tuning; restored on exit
+        KernelStats &k;
+        bool saved;
+        explicit StatsOff(KernelStats &ks) : k(ks), saved(ks.enabled) { k.enabled = false; }
+        ~StatsOff() { k.enabled = saved; }
+    } stats_off(kstats);
+    assemble(gamma, 1, d_h, d_s);
+    const size_t ff = factor_floats(), dif = (size_t)ts.n_diag * 2 * TILE_FLOATS;
+    float *tiles = buf<float>("tiles", (size_t)nmat * ff);
+    float *dinv = buf<float>("dinv", (size_t)nmat * dif);
+    int *repl = buf<int>("replaced", 65536);
+    // shifts on the imaginary axis: the factorisation's cost does not depend
+    // on the values (static pivoting, fixed structure)
+    std::vector<int32_t> kidx(nmat, 0);
+    std::vector<float2> z(nmat);
+    for (int j = 0; j < nmat; ++j) z[j] = make_float2(0.f, 1.f + (float)j);
+    factor_ms.assign(candidates.size(), 0.0);
+    int best = candidates.empty() ? schur_group : candidates[0];
+    double best_ms = 1e300;
+    for (size_t c = 0; c < candidates.size(); ++c) {
+        plan_schur(candidates[c]);
+        factor(nmat, kidx, z, precision, tiles, dinv, repl);  // captures the sequence
+        AOSX_HIP(hipStreamSynchronize(stream));
+        const double t0 = now();
+        for (int r = 0; r < repeats; ++r) factor(nmat, kidx, z, precision, tiles, dinv, repl);
+        AOSX_HIP(hipStreamSynchronize(stream));
+        factor_ms[c] = (now() - t0) / repeats * 1e3;
+        if (factor_ms[c] < best_ms) {
+            best_ms = factor_ms[c];
+            best = candidates[c];
+        }
+    }
+    plan_schur(best);
+    return best;
 }
 
 void Bundle::spmm(float2 *y, const float2 *a, const int32_t *d_kidx, const float2 *x, const int32_t *d_xidx, int w,
@@ -362,20 +518,15 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
     const float2 *d_s = static_cast<const float2 *>(ws.at("S").ptr);
     int32_t *d_kidx = buf<int32_t>("f_kidx", nmat);
     float2 *d_z = buf<float2>("f_z", nmat);
+    const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
+    uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
     AOSX_HIP(hipMemcpyAsync(d_kidx, kidx.data(), nmat * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_z, z.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
     const int64_t mstride = (int64_t)factor_floats();
-    const int64_t count4 = (int64_t)nmat * mstride / 4;
-    k_fill_zero<<<blocks_for(count4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(tiles), count4);
-    AOSX_LAUNCHED();
-    k_scatter_shifted<<<dim3(blocks_for(nnz, 256, 4096), nmat), 256, 0, stream>>>(
-        tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, n);
-    AOSX_LAUNCHED();
     const int nt = ts.n_diag;
     const float pivot_rel = 1.0e-6f;
     const int ng = (int)grp_begin.size() - 1;
-    const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
-    uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
+    constexpr double TILE_FLOPS = 8.0 * 64.0 * 64.0 * 64.0;  // one complex 64^3 tile update
     // A(i,j) -= sum_{t in [t0, t1)} L(i,t) U(t,j) for the listed pairs, the
     // target held in the accumulators across the steps
     auto multi_update = [&](int t0, int t1, int64_t first, int64_t np) {
@@ -392,63 +543,66 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat, d_tile_id);
         AOSX_LAUNCHED();
     };
-    for (int g = 0; g < ng; ++g) {
-        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-        int next_slot = 0;  // split slots are handed out step by step within the group
-        for (int k = g0; k < g1; ++k) {
-            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
-            const int slot0 = next_slot;  // split slot of this step's first panel tile
-            next_slot += 2 * s;
-            const int32_t *nx = d_next + ts.next_ptr[k];
-            // row and column k take the updates of the group's earlier steps
-            const int64_t nc = crout_ptr[k + 1] - crout_ptr[k];
-            if (nc > 0) {
-                EventTimer tm(kstats.enabled, stream);
-                multi_update(g0, k, crout_ptr[k], nc);
-                if (kstats.enabled) {
-                    kstats.inner_ms += tm.stop();
-                    kstats.inner_launches++;
-                    kstats.inner_flops += 8.0 * 64.0 * 64.0 * 64.0 * crout_updates[k] * nmat;
+    auto launches = [&]() {
+        const int64_t count4 = (int64_t)nmat * mstride / 4;
+        k_fill_zero<<<blocks_for(count4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(tiles), count4);
+        AOSX_LAUNCHED();
+        k_scatter_shifted<<<dim3(blocks_for(nnz, 256, 4096), nmat), 256, 0, stream>>>(
+            tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, n);
+        AOSX_LAUNCHED();
+        for (int g = 0; g < ng; ++g) {
+            const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+            int next_slot = 0;  // split slots are handed out step by step within the group
+            for (int k = g0; k < g1; ++k) {
+                const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+                const int slot0 = next_slot;  // split slot of this step's first panel tile
+                next_slot += 2 * s;
+                const int32_t *nx = d_next + ts.next_ptr[k];
+                // row and column k take the updates of the group's earlier steps
+                const int64_t nc = crout_ptr[k + 1] - crout_ptr[k];
+                if (nc > 0) {
+                    const int r = range_begin(RANGE_INNER);
+                    multi_update(g0, k, crout_ptr[k], nc);
+                    range_end(r, TILE_FLOPS * crout_updates[k] * nmat);
                 }
-            }
-            {
-                EventTimer tm(kstats.enabled, stream);
+                int r = range_begin(RANGE_GETRF);
                 k_getrf_diag<<<nmat, 256, 0, stream>>>(tiles, mstride, dinv, nt, k, d_tile_id, d_replaced,
                                                        pivot_rel);
                 AOSX_LAUNCHED();
-                if (kstats.enabled) { kstats.getrf_ms += tm.stop(); kstats.getrf_launches++; }
+                range_end(r, 0.0);
+                if (s == 0) continue;
+                r = range_begin(RANGE_PANEL);
+                k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id,
+                                                               split, split_tiles, slot0);
+                AOSX_LAUNCHED();
+                range_end(r, 0.0);
             }
-            if (s == 0) continue;
-            EventTimer tm(kstats.enabled, stream);
-            k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id, split,
-                                                           split_tiles, slot0);
-            AOSX_LAUNCHED();
-            if (kstats.enabled) { kstats.panel_ms += tm.stop(); kstats.panel_launches++; }
+            // every target outside the group's rows and columns, once
+            const int64_t np = grp_pair_ptr[g + 1] - grp_pair_ptr[g];
+            if (np == 0) continue;
+            const int r = range_begin(RANGE_SCHUR);
+            if (split_mode) {
+                const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
+                const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
+                const int32_t *qd = d_quads + 4 * grp_quad_ptr[g];
+                if (precision == AOSX_PRECISION_BF16X3_IR)
+                    k_schur_quad_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq,
+                                                                         nmat, d_tile_id, split, split_tiles, d_ppos,
+                                                                         d_step_s);
+                else
+                    k_schur_quad_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq,
+                                                                          nmat, d_tile_id, split, split_tiles,
+                                                                          d_ppos, d_step_s);
+                AOSX_LAUNCHED();
+            } else {
+                multi_update(g0, g1, grp_pair_ptr[g], np);
+            }
+            range_end(r, TILE_FLOPS * grp_updates[g] * nmat);
         }
-        // every target outside the group's rows and columns, once
-        const int64_t np = grp_pair_ptr[g + 1] - grp_pair_ptr[g];
-        if (np == 0) continue;
-        EventTimer tm(kstats.enabled, stream);
-        if (split_mode) {
-            const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
-            const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
-            const int32_t *qd = d_quads + 4 * grp_quad_ptr[g];
-            if (precision == AOSX_PRECISION_BF16X3_IR)
-                k_schur_quad_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat,
-                                                                     d_tile_id, split, split_tiles, d_ppos, d_step_s);
-            else
-                k_schur_quad_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat,
-                                                                      d_tile_id, split, split_tiles, d_ppos, d_step_s);
-            AOSX_LAUNCHED();
-        } else {
-            multi_update(g0, g1, grp_pair_ptr[g], np);
-        }
-        if (kstats.enabled) {
-            kstats.schur_ms += tm.stop();
-            kstats.schur_launches++;
-            kstats.schur_flops += 8.0 * 64.0 * 64.0 * 64.0 * grp_updates[g] * nmat;
-        }
-    }
+    };
+    run_sequence({0, nmat, precision, (int64_t)tiles, (int64_t)dinv, (int64_t)d_replaced, (int64_t)split,
+                  (int64_t)d_h, (int64_t)d_s, (int64_t)d_kidx, (int64_t)d_z},
+                 launches);
 }
 
 // ------------------------------------------------------------------- solves
@@ -472,77 +626,77 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
                                                                     ctrans ? 1 : 0);
         AOSX_LAUNCHED();
     };
-    EventTimer tm(kstats.enabled, stream);
     // Step groups pay off when a launch covers many matrices; with a few, the
     // per-step sweep (more, shorter launches) keeps more of the GPU busy.
     const bool grouped = nmat >= 8;
-    if (!grouped) {
-        for (int k = 0; k < nt; ++k) {
-            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
-            AOSX_LAUNCHED();
-            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
-            if (s) {
-                update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                          d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+    // algorithmic bytes: every stored tile is read once per sweep pair,
+    // every RHS block is read and written once per tile it meets
+    const double bytes = (double)nmat * ((double)ts.n_tiles * TILE_FLOATS * 4.0 +
+                                         2.0 * (double)ts.next_idx.size() * nchunk * TILE_FLOATS * 4.0 * 3.0);
+    auto launches = [&]() {
+        const int r = range_begin(RANGE_SOLVE);
+        if (!grouped) {
+            for (int k = 0; k < nt; ++k) {
+                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
                 AOSX_LAUNCHED();
+                const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+                if (s) {
+                    update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+                    AOSX_LAUNCHED();
+                }
+            }
+            for (int k = nt - 1; k >= 0; --k) {
+                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
+                AOSX_LAUNCHED();
+                const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
+                if (s) {
+                    update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
+                    AOSX_LAUNCHED();
+                }
             }
         }
-        for (int k = nt - 1; k >= 0; --k) {
-            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
-            AOSX_LAUNCHED();
-            const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
-            if (s) {
-                update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                          d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
+        // forward: Y_k <- D_k Y_k, then the rows of the group below k, then the
+        // group's trailing rows in one launch
+        for (int g = 0; grouped && g < ng; ++g) {
+            const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+            for (int k = g0; k < g1; ++k) {
+                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
                 AOSX_LAUNCHED();
+                const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
+                const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
+                int p = 0;
+                while (p < s && nx[p] < g1) ++p;
+                if (p) {
+                    update<<<dim3(p, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+                    AOSX_LAUNCHED();
+                }
             }
+            trailing(g0, g1, d_fwd_rows + fwd_row_ptr[g], fwd_row_ptr[g + 1] - fwd_row_ptr[g]);
         }
-    }
-    // forward: Y_k <- D_k Y_k, then the rows of the group below k, then the
-    // group's trailing rows in one launch
-    for (int g = 0; grouped && g < ng; ++g) {
-        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-        for (int k = g0; k < g1; ++k) {
-            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
-            AOSX_LAUNCHED();
-            const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
-            const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
-            int p = 0;
-            while (p < s && nx[p] < g1) ++p;
-            if (p) {
-                update<<<dim3(p, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                          d_next + ts.next_ptr[k], ctrans ? 1 : 0);
+        // backward: the same groups in reverse, steps descending
+        for (int g = ng - 1; grouped && g >= 0; --g) {
+            const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
+            for (int k = g1 - 1; k >= g0; --k) {
+                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
                 AOSX_LAUNCHED();
+                const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
+                const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
+                int a = s;
+                while (a > 0 && pv[a - 1] >= g0) --a;
+                if (a < s) {
+                    update<<<dim3(s - a, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
+                                                                  d_prev + ts.prev_ptr[k] + a, ctrans ? 1 : 0);
+                    AOSX_LAUNCHED();
+                }
             }
+            trailing(g0, g1, d_bwd_rows + bwd_row_ptr[g], bwd_row_ptr[g + 1] - bwd_row_ptr[g]);
         }
-        trailing(g0, g1, d_fwd_rows + fwd_row_ptr[g], fwd_row_ptr[g + 1] - fwd_row_ptr[g]);
-    }
-    // backward: the same groups in reverse, steps descending
-    for (int g = ng - 1; grouped && g >= 0; --g) {
-        const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-        for (int k = g1 - 1; k >= g0; --k) {
-            diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
-            AOSX_LAUNCHED();
-            const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
-            const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
-            int a = s;
-            while (a > 0 && pv[a - 1] >= g0) --a;
-            if (a < s) {
-                update<<<dim3(s - a, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_prev + ts.prev_ptr[k] + a, ctrans ? 1 : 0);
-                AOSX_LAUNCHED();
-            }
-        }
-        trailing(g0, g1, d_bwd_rows + bwd_row_ptr[g], bwd_row_ptr[g + 1] - bwd_row_ptr[g]);
-    }
-    if (kstats.enabled) {
-        kstats.solve_ms += tm.stop();
-        kstats.solve_launches += 1;
-        // algorithmic bytes: every stored tile is read once per sweep pair,
-        // every RHS block is read and written once per tile it meets
-        kstats.solve_bytes += (double)nmat * ((double)ts.n_tiles * TILE_FLOATS * 4.0 +
-                                              2.0 * (double)ts.next_idx.size() * nchunk * TILE_FLOATS * 4.0 * 3.0);
-    }
+        range_end(r, bytes);
+    };
+    run_sequence({1, nmat, W, (int64_t)y, (int64_t)tiles, (int64_t)dinv, ctrans ? 1 : 0, precision}, launches);
 }
 
 void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,

@@ -142,9 +142,13 @@ class BandServer:
         self.batch_log: list[dict] = []
 
     # ----------------------------------------------------------- bundles
-    def register(self, key: str, bundle: LocalizedOperatorBundle) -> dict:
-        """Analyse and upload a bundle once; it stays resident."""
+    def register(self, key: str, bundle: LocalizedOperatorBundle, *, autotune: CissSettings | None = None) -> dict:
+        """Analyse and upload a bundle once; it stays resident.  With
+        ``autotune`` the bundle's step-group size is tuned for batches of
+        those settings (``k_batch`` k points) before it takes jobs."""
         native = NativeBundle(bundle, device=self.device)
+        if autotune is not None:
+            native.autotune(autotune.factorizations_per_launch, autotune.precision)
         res = _Resident(key, native)
         with self._lock:
             if key in self._bundles:

@@ -149,6 +149,32 @@ class NativeBundle:
             raise SolverError("cannot release the bundle workspace")
         return int(held.value)
 
+    def autotune(
+        self, matrices: int, precision: str = "bf16x3_ir", candidates: tuple[int, ...] = (), repeats: int = 1
+    ) -> dict:
+        """Time factorisations of ``matrices`` shifted matrices for each
+        step-group size (default {4, 8, 12, 16, 24, 32}) and keep the fastest
+        for this bundle.  Returns the chosen size and the per-candidate
+        milliseconds per factorisation launch."""
+        req = N.new(N.AutotuneRequest)
+        req.matrices = int(matrices)
+        req.precision = N.PRECISION[precision]
+        req.repeats = int(repeats)
+        cand = np.ascontiguousarray(np.asarray(candidates, dtype=np.int32))
+        req.n_candidates = cand.size
+        if cand.size:
+            req.candidates = N.ptr(cand, C.c_int32)
+        res = N.AutotuneResult()
+        rc = N.load().aosx_bundle_autotune_v1(self.handle, C.byref(req), C.byref(res))
+        N.check(rc, res, "step-group autotuning")
+        n = int(res.n_candidates)
+        return {
+            "chosen_group": int(res.chosen_group),
+            "factor_ms": {int(res.candidates[i]): float(res.factor_ms[i]) for i in range(n)},
+            "matrices": int(matrices),
+            "precision": precision,
+        }
+
     def kernel_stats(self, enable: bool) -> dict:
         out = N.KernelStats()
         N.load().aosx_kernel_stats_read_v1(self.handle, 1 if enable else 0, C.byref(out))

@@ -306,6 +306,39 @@ typedef struct {
 
 int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_stats_v1 *stats);
 
+/* ------------------------------------------------------------- autotuning
+ * Picks the step-group size of the bundle's tile LU (how many elimination
+ * steps a trailing Schur launch accumulates, and the group the solve sweeps
+ * use) by timing factorisations of `matrices` shifted copies of H(Gamma),
+ * S(Gamma) for each candidate; keeps the fastest for every later call.  No
+ * counterpart in the reference (cuDSS chooses its own schedule); the
+ * factorisations use the workspace a band solve of that size would use. */
+#define AOSX_AUTOTUNE_MAX_CANDIDATES 16
+typedef struct {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    int32_t matrices;              /* factorisations per launch to tune for, 1..256 */
+    int32_t precision;             /* AOSX_PRECISION_*                             */
+    int32_t n_candidates;          /* 0: the built-in set {4, 8, 12, 16, 24, 32}  */
+    int32_t repeats;               /* timed factorisations per candidate, >= 1     */
+    const int32_t *candidates;     /* [n_candidates] group sizes in 1..32          */
+} aosx_autotune_request_v1;
+
+typedef struct {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    int32_t status;
+    int32_t hip_status;
+    int32_t chosen_group;
+    int32_t n_candidates;
+    int32_t candidates[AOSX_AUTOTUNE_MAX_CANDIDATES];
+    double factor_ms[AOSX_AUTOTUNE_MAX_CANDIDATES];  /* mean per factorisation launch */
+    char error_message[AOSX_ERROR_CAPACITY];
+} aosx_autotune_result_v1;
+
+int aosx_bundle_autotune_v1(aosx_bundle *bundle, const aosx_autotune_request_v1 *request,
+                            aosx_autotune_result_v1 *result);
+
 /* sizeof of every request/result struct, in declaration order, so bindings
  * can verify their layouts; returns the number of sizes written. */
 int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity);

@@ -144,7 +144,9 @@ directory = "out"
 def test_read_band_config(tmp_path):
     cfg = read_band_config(_toml(tmp_path, 'precision = "bf16x3_ir"\nk_batch = 4\nconjugate_pairs = true\n'
                                            "refinement_passes = 2\nrefinement_steps = 2\n"
-                                           'refinement_policy = "on_demand"\nresidual_target_ev = 5e-5'))
+                                           'refinement_policy = "on_demand"\nresidual_target_ev = 5e-5\n'
+                                           "autotune = true"))
+    assert cfg.settings.autotune and cfg.settings.factorizations_per_launch == 4 * 16
     assert cfg.settings.refinement_policy == "on_demand" and cfg.settings.residual_target_ev == 5e-5
     assert cfg.settings.contour_batch_size == 16 and cfg.settings.integration_points == 32
     assert cfg.settings.precision == "bf16x3_ir" and cfg.settings.k_batch == 4 and cfg.settings.conjugate_pairs

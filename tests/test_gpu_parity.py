@@ -181,6 +181,66 @@ def test_on_demand_refinement_resolves_missed_targets(native_small):
         assert a.maximum_residual_ev == pytest.approx(b.maximum_residual_ev, rel=1e-6)
 
 
+def test_graph_replays_match_eager_launches(small2d):
+    """Factorisations and solves replayed from captured HIP graphs give the
+    eager launches' results, replay after replay, with eager work and
+    synchronisation in between; kernel timing works inside the graphs."""
+    kpts = [np.array([[0.13, 0.37, 0.0]]), np.array([[0.25, 0.1, 0.0]]), np.array([[0.0, 0.5, 0.0]])]
+    interval = EnergyInterval(-2.02, 2.02)
+    settings = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision="bf16x3_ir",
+                            refinement_steps=1, refinement_passes=2)
+    os.environ["AOSX_GRAPHS"] = "0"
+    try:
+        eager = NativeBundle(small2d, device=0)
+    finally:
+        del os.environ["AOSX_GRAPHS"]
+    graphed = NativeBundle(small2d, device=0)
+    try:
+        for rep in range(2):
+            for k in kpts:
+                say("graph replay", rep, k[0])
+                (a,), _ = band_solve(graphed, settings, interval, kpoints=k)
+                (b,), _ = band_solve(eager, settings, interval, kpoints=k)
+                assert a.energies_ev.size == b.energies_ev.size
+                assert np.max(np.abs(a.energies_ev - b.energies_ev), initial=0.0) <= 1e-6
+                assert a.maximum_residual_ev == pytest.approx(b.maximum_residual_ev, rel=1e-5)
+        graphed.kernel_stats(True)
+        band_solve(graphed, settings, interval, kpoints=kpts[0])
+        band_solve(graphed, settings, interval, kpoints=kpts[1])
+        ks = graphed.kernel_stats(False)
+        say("graph kernel stats", {k: v for k, v in ks.items() if k.endswith("launches")})
+        # small2d has 13 tile columns: one step group, so no trailing launches
+        assert ks["inner_launches"] > 0 and ks["inner_ms"] > 0.0
+        assert ks["getrf_launches"] > 0 and ks["solve_launches"] > 0 and ks["solve_ms"] > 0.0
+    finally:
+        graphed.close()
+        eager.close()
+
+
+def test_autotune_picks_a_timed_group_and_keeps_results(small2d, small2d_pattern):
+    """The autotuner times every candidate step-group size and keeps the
+    fastest; solves under each plan match the FP64 oracle."""
+    nb = NativeBundle(small2d, device=0)
+    try:
+        k = np.array([[0.13, 0.37, 0.0]])
+        interval = EnergyInterval(-2.02, 2.02)
+        settings = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision="bf16x3_ir",
+                                refinement_steps=1, refinement_passes=2)
+        ref = _oracle_window(small2d, small2d_pattern, k[0], interval.lower_ev, interval.upper_ev)
+        for cands in ((1,), (4,), (1, 4, 16)):
+            say("autotune", cands)
+            tune = nb.autotune(settings.factorizations_per_launch, "bf16x3_ir", candidates=cands)
+            assert tune["chosen_group"] in cands
+            assert set(tune["factor_ms"]) == set(cands) and all(v > 0 for v in tune["factor_ms"].values())
+            assert tune["chosen_group"] == min(tune["factor_ms"], key=tune["factor_ms"].get)
+            (r,), _ = band_solve(nb, settings, interval, kpoints=k)
+            assert r.energies_ev.size == ref.size
+            assert np.max(np.abs(np.sort(r.energies_ev) - ref)) < 1e-4
+            assert r.maximum_residual_ev <= 1e-4
+    finally:
+        nb.close()
+
+
 def test_band_solve_matches_oracle_ciss(small2d, small2d_pattern, native_small):
     """Single-pass plain CISS: same energies and rank as the CPU restatement
     of the reference algorithm (same probes, same settings)."""
