@@ -229,6 +229,7 @@ class KernelStats(C.Structure):
         ("inner_launches", C.c_int64),
         ("inner_flops", C.c_double),
         ("inner_ms", C.c_double),
+        ("schur_bytes", C.c_double),
     ]
 
 

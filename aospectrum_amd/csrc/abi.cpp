@@ -407,6 +407,7 @@ int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_s
     out->inner_launches = k.inner_launches;
     out->inner_flops = k.inner_flops;
     out->inner_ms = k.inner_ms;
+    out->schur_bytes = k.schur_bytes;
     b.kstats = aosx::KernelStats();
     b.kstats.enabled = enable != 0;
     return 0;

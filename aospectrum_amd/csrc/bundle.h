@@ -35,6 +35,7 @@ struct KernelStats {
     int64_t schur_launches = 0;     // trailing (grouped) Schur updates
     double schur_flops = 0.0;
     double schur_ms = 0.0;
+    double schur_bytes = 0.0;       // algorithmic bytes of the trailing launches
     int64_t inner_launches = 0;     // Schur updates inside a step group
     double inner_flops = 0.0;
     double inner_ms = 0.0;
@@ -59,6 +60,7 @@ struct TimedRange {
     hipEvent_t a = nullptr, b = nullptr;
     int kind = 0;
     double work = 0.0;
+    double bytes = 0.0;
 };
 
 // A captured launch sequence: the executable graph and the timed ranges whose
@@ -109,6 +111,8 @@ struct Bundle {
     std::vector<int32_t> grp_begin;
     std::vector<int64_t> grp_pair_ptr;
     std::vector<double> grp_updates;  // tile updates of each trailing launch
+    std::vector<double> grp_tiles;    // tiles a trailing launch must move per matrix:
+                                      // each target read + written, each operand read
     int32_t *d_pairs = nullptr;
     // the same trailing targets as 2 x 2 blocks (i0, i1, j0, j1; -1 absent)
     std::vector<int64_t> grp_quad_ptr;
@@ -140,7 +144,7 @@ struct Bundle {
     std::vector<TimedRange> *ranges = nullptr;  // collecting ranges of the running sequence
     bool capturing = false;
     int range_begin(int kind);
-    void range_end(int id, double work);
+    void range_end(int id, double work, double bytes = 0.0);
     void flush_ranges(std::vector<TimedRange> &rs, bool destroy);
     void clear_graphs();
     template <typename F>

@@ -75,11 +75,12 @@ int Bundle::range_begin(int kind) {
     return (int)ranges->size() - 1;
 }
 
-void Bundle::range_end(int id, double work) {
+void Bundle::range_end(int id, double work, double bytes) {
     if (id < 0) return;
     TimedRange &r = (*ranges)[id];
     AOSX_HIP(hipEventRecordWithFlags(r.b, stream, capturing ? hipEventRecordExternal : 0));
     r.work = work;
+    r.bytes = bytes;
 }
 
 // Waits for the last range, adds every range to the kernel statistics.
@@ -90,7 +91,12 @@ void Bundle::flush_ranges(std::vector<TimedRange> &rs, bool destroy) {
         float ms = 0.f;
         AOSX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
         switch (r.kind) {
-        case RANGE_SCHUR: kstats.schur_ms += ms; kstats.schur_launches++; kstats.schur_flops += r.work; break;
+        case RANGE_SCHUR:
+            kstats.schur_ms += ms;
+            kstats.schur_launches++;
+            kstats.schur_flops += r.work;
+            kstats.schur_bytes += r.bytes;
+            break;
         case RANGE_INNER: kstats.inner_ms += ms; kstats.inner_launches++; kstats.inner_flops += r.work; break;
         case RANGE_PANEL: kstats.panel_ms += ms; kstats.panel_launches++; break;
         case RANGE_GETRF: kstats.getrf_ms += ms; kstats.getrf_launches++; break;
@@ -295,6 +301,7 @@ void Bundle::plan_schur(int group) {
     grp_begin.clear();
     grp_pair_ptr.assign(1, 0);
     grp_updates.clear();
+    grp_tiles.clear();
     std::vector<int32_t> pairs, quads;
     grp_quad_ptr.assign(1, 0);
     std::vector<uint8_t> mark((size_t)nt * nt, 0);
@@ -303,7 +310,7 @@ void Bundle::plan_schur(int group) {
     for (int g0 = 0; g0 < nt; g0 += group) {
         const int g1 = std::min(nt, g0 + group);
         grp_begin.push_back(g0);
-        double updates = 0.0;
+        double updates = 0.0, operands = 0.0;
         int64_t slot = 0;
         std::vector<int32_t> rows;  // tiles >= g1 touched by the group, ascending
         for (int t = g0; t < g1; ++t) {
@@ -315,6 +322,7 @@ void Bundle::plan_schur(int group) {
             int p = 0;
             while (p < s && nx[p] < g1) ++p;
             updates += (double)(s - p) * (s - p);
+            operands += 2.0 * (s - p);
             for (int a = p; a < s; ++a) {
                 rows.push_back(nx[a]);
                 for (int b = p; b < s; ++b) mark[(size_t)nx[a] * nt + nx[b]] = 1;
@@ -348,6 +356,7 @@ void Bundle::plan_schur(int group) {
             for (int b = 0; b < r; ++b) mark[(size_t)rows[a] * nt + rows[b]] = 0;
         grp_pair_ptr.push_back((int64_t)pairs.size() / 2);
         grp_quad_ptr.push_back((int64_t)quads.size() / 4);
+        grp_tiles.push_back(2.0 * (double)(grp_pair_ptr.back() - grp_pair_ptr[grp_pair_ptr.size() - 2]) + operands);
         grp_updates.push_back(updates);
         split_tiles = std::max(split_tiles, slot);
     }
@@ -597,7 +606,7 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             } else {
                 multi_update(g0, g1, grp_pair_ptr[g], np);
             }
-            range_end(r, TILE_FLOPS * grp_updates[g] * nmat);
+            range_end(r, TILE_FLOPS * grp_updates[g] * nmat, grp_tiles[g] * nmat * TILE_FLOATS * 4.0);
         }
     };
     run_sequence({0, nmat, precision, (int64_t)tiles, (int64_t)dinv, (int64_t)d_replaced, (int64_t)split,

@@ -43,13 +43,15 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0}
 
-# HBM bytes per k_schur dispatch (workload, precision) -> bytes, from two
+# HBM bytes per trailing Schur dispatch (workload, precision) -> bytes, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this bench, scaled by what
 # the counters read for a copy kernel of known size; the commands and the
-# per-kernel table are in profiles/k_schur_bulk3d_traffic.md.  Counters
+# per-kernel table are in profiles/k_schur_quad_bulk3d_traffic.md.  Counters
 # cannot be read from inside a timed run, so the figure is carried here and
 # refreshed whenever the kernel changes.
-MEASURED_SCHUR_TRAFFIC: dict = {}
+MEASURED_SCHUR_TRAFFIC = {("bulk3d", "bf16x3_ir"): 135.66e9}
+# BF16 split products issued per algorithmic product (hi*hi + hi*lo + lo*hi)
+MFMA_PRODUCTS = {"fp32": 1, "fp32_ir": 1, "bf16x3_ir": 3, "bf16_ir": 1}
 
 
 def log(*parts):
@@ -391,7 +393,8 @@ def main() -> int:
         "factor_flops_per_matrix": info["factor_flops"],
         "roofline": {
             "bound": "mfma",
-            "kernel": f"k_schur_group ({args.precision}, trailing update of each step group)",
+            "kernel": ("k_schur_quad_split" if args.precision.startswith("bf16") else "k_schur_group")
+            + f" ({args.precision}, trailing update of each step group)",
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",
@@ -400,6 +403,8 @@ def main() -> int:
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-scaled)",
             "launches": ks["schur_launches"],
             "avg_launch_ms": ks["schur_ms"] / max(ks["schur_launches"], 1),
+            "algorithmic_bytes_per_launch": ks["schur_bytes"] / max(ks["schur_launches"], 1),
+            "mfma_issued_tflops": achieved * MFMA_PRODUCTS[args.precision],
         },
         "kernel_ms": {k: v for k, v in ks.items() if k.endswith("_ms")},
         "cpu_baseline": cpu,

@@ -302,6 +302,8 @@ typedef struct {
     int64_t inner_launches;        /* Schur updates inside a step group            */
     double inner_flops;
     double inner_ms;
+    double schur_bytes;            /* algorithmic bytes of the trailing launches:  */
+                                   /* targets read + written once, operands once   */
 } aosx_kernel_stats_v1;
 
 int aosx_kernel_stats_read_v1(aosx_bundle *bundle, int32_t enable, aosx_kernel_stats_v1 *stats);
