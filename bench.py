@@ -116,7 +116,8 @@ def serving_bench(args) -> int:
         half = float(rng.uniform(0.5, 2.0))
         block = int(rng.choice([32, 64]))
         st = CissSettings(block_size=block, contour_batch_size=16, conjugate_pairs=True,
-                          precision=args.precision, refinement_steps=args.ir_steps, refinement_passes=args.passes)
+                          precision=args.precision, refinement_steps=args.ir_steps, refinement_passes=args.passes,
+                          refinement_policy=args.refinement_policy)
         jobs.append((t, BandJob(name, path.fractional_points[idx], EnergyInterval(-half - 0.02, half + 0.02), st,
                                 f"job{j}-{name}-{idx.size}k-h{half:.2f}-L{block}")))
     log(f"serving: {len(jobs)} jobs over {[n for n in names]}, {sum(j.kpoints.shape[0] for _, j in jobs)} k points")
@@ -127,8 +128,11 @@ def serving_bench(args) -> int:
             f"{entry['end'] - entry['start']:.2f} s" + (f", error {entry['error'][:80]}" if entry["error"] else ""))
 
     with BandServer(k_batch=args.k_batch, on_batch=progress) as server:
+        tune_for = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision=args.precision,
+                                k_batch=args.k_batch)
         for name, b in bundles.items():
-            server.register(name, b)
+            server.register(name, b, autotune=None if args.no_autotune else tune_for)
+            log(f"registered {name}")
         t0 = time.perf_counter()
         futures = []
         for arrival, job in jobs:
@@ -185,6 +189,7 @@ def serving_bench(args) -> int:
             "workload": f"Poisson {args.rate} jobs/s, {args.jobs} jobs, bundles {names}, k-path lengths "
             f"{{20,61,121}} x {args.kscale}, half-width U[0.5,2] eV, block {{32,64}}",
             "precision": args.precision,
+            "refinement_policy": args.refinement_policy,
             "k_batch": args.k_batch,
         },
         "data": "synthetic (seeded generator)",
