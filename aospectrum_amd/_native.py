@@ -185,7 +185,7 @@ class BandRequest(C.Structure):
         ("refinement_passes", C.c_int32),
         ("k_batch", C.c_int32),
         ("conjugate_pairs", C.c_int32),
-        ("reserved", C.c_int32),
+        ("resident_limit", C.c_int32),
         ("energies_ev", _F32P),
         ("energy_capacity", C.c_int32),
         ("energy_counts", _I32P),

@@ -14,7 +14,8 @@ Settings beyond the reference's (all default to the reference's behaviour):
 ``refinement_passes`` (extra contour-filter passes over the Ritz block),
 ``k_batch`` (k points factored together), ``conjugate_pairs`` (solve the
 mirrored shift with the conjugate-transposed factors), ``refinement_policy``
-(``fixed`` | ``on_demand``), ``residual_target_ev`` and ``autotune``.
+(``fixed`` | ``on_demand``), ``residual_target_ev``, ``autotune`` and
+``resident_limit``.
 """
 
 from __future__ import annotations
@@ -83,6 +84,10 @@ class CissSettings:
     # time the step-group sizes of the tile LU on the bundle before the first
     # solve and keep the fastest (NativeBundle.autotune)
     autotune: bool = False
+    # filter passes keep the factorisations of a batch resident when at most
+    # this many (0: no limit) and they fit in device memory; otherwise every
+    # pass factors each shift group again (streamed shifts)
+    resident_limit: int = 0
 
     @property
     def factorizations_per_launch(self) -> int:
@@ -100,7 +105,7 @@ class CissSettings:
             raise InputError("CISS sizes must be positive and contour_batch_size must divide integration_points")
         if self.precision not in N.PRECISION:
             raise InputError(f"precision must be one of {sorted(N.PRECISION)}")
-        if self.refinement_steps < 0 or self.refinement_passes < 0 or self.k_batch < 1:
+        if self.refinement_steps < 0 or self.refinement_passes < 0 or self.k_batch < 1 or self.resident_limit < 0:
             raise InputError("refinement counts must be >= 0 and k_batch >= 1")
         if self.conjugate_pairs and self.integration_points % 2:
             raise InputError("conjugate_pairs needs an even number of integration points")
@@ -270,6 +275,7 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
     req.refinement_passes = settings.refinement_passes
     req.k_batch = settings.k_batch
     req.conjugate_pairs = 1 if settings.conjugate_pairs else 0
+    req.resident_limit = settings.resident_limit
     req.energies_ev = N.ptr(energies, C.c_float)
     req.energy_capacity = cap
     req.energy_counts = N.ptr(counts, C.c_int32)

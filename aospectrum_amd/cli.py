@@ -12,7 +12,8 @@ k points over one spawned process per GPU, as the reference does
 
 Extra ``[solver]`` keys (defaults keep the reference's behaviour):
 ``precision``, ``refinement_steps``, ``refinement_passes``, ``k_batch``,
-``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``, ``autotune``.
+``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``, ``autotune``,
+``resident_limit``.
 """
 
 from __future__ import annotations
@@ -137,6 +138,7 @@ def read_band_config(path: str | Path) -> BandConfig:
             refinement_policy=str(solver.get("refinement_policy", "fixed")),
             residual_target_ev=float(solver.get("residual_target_ev", 1.0e-4)),
             autotune=tune,
+            resident_limit=int(solver.get("resident_limit", 0)),
         )
     except (TypeError, ValueError) as exc:
         raise InputError("invalid Band solver setting") from exc

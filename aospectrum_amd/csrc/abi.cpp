@@ -362,6 +362,7 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         p.refinement_passes = q->refinement_passes;
         p.k_batch = q->k_batch;
         p.conjugate_pairs = q->conjugate_pairs;
+        p.resident_limit = q->resident_limit;
         p.intervals = q->intervals;
         aosx::CissOutput o;
         b.band(p, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),

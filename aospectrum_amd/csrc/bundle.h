@@ -77,6 +77,7 @@ struct CissParams {
     double vscale = 0.1, guard = 1e-5, rank_tol = 1e-5, spurious = 1e-4;
     int precision = AOSX_PRECISION_FP32, refinement_steps = 0, refinement_passes = 0, k_batch = 1;
     int conjugate_pairs = 0;
+    int resident_limit = 0;  // 0: keep factorisations resident whenever they fit
     const double *intervals = nullptr;  // [nk][2] per-k absolute intervals, or null
 };
 

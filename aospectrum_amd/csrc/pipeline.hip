@@ -1060,7 +1060,6 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     const int K = std::max(1, p.k_batch);
     const int nfac = p.conjugate_pairs ? N / 2 : N;
     const int cb = std::max(1, std::min(p.contour_batch_size, nfac));
-    const bool resident = p.refinement_passes > 0;
     const bool ir = p.precision != AOSX_PRECISION_FP32;
     const int ir_steps = ir ? std::max(0, p.refinement_steps) : 0;
     // refinement schedule: with filter passes ahead, the moment pass and the
@@ -1068,6 +1067,14 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     // step there, the full count on the last filter application
     const int ir_first = p.refinement_passes > 0 ? std::min(ir_steps, 1) : ir_steps;
     int *d_repl = buf<int>("replaced", 65536);
+    // AOSX_PROGRESS: one line per factorisation group and pass on stderr
+    const bool progress = std::getenv("AOSX_PROGRESS") != nullptr;
+    auto report = [&](int64_t kb, int pass, int g, int ng) {
+        if (!progress) return;
+        std::fprintf(stderr, "[aosx] k %lld pass %d group %d/%d done (%.1f s)\n", (long long)kb, pass, g + 1, ng,
+                     now() - t_start);
+        std::fflush(stderr);
+    };
     out.stages[1] += now() - t_start;
     for (int64_t kb = 0; kb < nk_total; kb += K) {
         const int Kc = (int)std::min<int64_t>(K, nk_total - kb);
@@ -1098,9 +1105,25 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
         // factorisation groups: all k of the batch x cb shifts
         const int G = Kc * cb;
         const int ngroups = (nfac + cb - 1) / cb;
-        const int slots = resident ? Kc * nfac : G;
         const size_t ff = factor_floats();
         const size_t dif = (size_t)ts.n_diag * 2 * 2 * 64 * 64;
+        // The filter passes reuse the factorisations when all of the batch's
+        // fit (with 8 GiB to spare for vectors and split panels) and the
+        // request allows that many; otherwise the factorisations are streamed:
+        // each pass factors every shift group again into one group's slots.
+        bool resident = p.refinement_passes > 0 && (p.resident_limit <= 0 || Kc * nfac <= p.resident_limit);
+        if (resident) {
+            size_t free_b = 0, total_b = 0;
+            AOSX_HIP(hipMemGetInfo(&free_b, &total_b));
+            size_t held = 0;
+            for (const char *name : {"tiles", "dinv"}) {
+                auto it = ws.find(name);
+                if (it != ws.end()) held += it->second.bytes;
+            }
+            const double need = (double)Kc * nfac * (double)(ff + dif) * 4.0 + 8.0 * (1ull << 30);
+            resident = need <= (double)(free_b + held);
+        }
+        const int slots = resident ? Kc * nfac : G;
         float *d_tiles = buf<float>("tiles", (size_t)slots * ff);
         float *d_dinv = buf<float>("dinv", (size_t)slots * dif);
         struct Group {
@@ -1165,6 +1188,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             }
             AOSX_HIP(hipStreamSynchronize(stream));
             out.stages[3] += now() - tb;
+            report(kb, 0, g, ngroups);
         }
         std::vector<int> repl(slots);
         AOSX_HIP(hipMemcpy(repl.data(), d_repl, slots * sizeof(int), hipMemcpyDeviceToHost));
@@ -1221,6 +1245,13 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 const int gsz = (int)gr.kidx.size();
                 float *tl = d_tiles + (size_t)gr.slot0 * ff;
                 float *di = d_dinv + (size_t)gr.slot0 * dif;
+                if (!resident) {  // streamed: factor this group again
+                    const double ta = now();
+                    factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0);
+                    AOSX_HIP(hipStreamSynchronize(stream));
+                    out.stages[2] += now() - ta;
+                    out.factorizations += gsz;
+                }
                 solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_pass, d_X, nullptr,
                            p.precision);
                 accumulate(gr, false, d_Y, W2, 1);
@@ -1228,6 +1259,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                     solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_pass, d_X,
                                nullptr, p.precision);
                     accumulate(gr, true, d_Y, W2, 1);
+                }
+                if (progress) {
+                    AOSX_HIP(hipStreamSynchronize(stream));
+                    report(kb, pass + 1, g, ngroups);
                 }
             }
             if (augment) {

@@ -107,6 +107,7 @@ def _settings_key(s: CissSettings) -> tuple:
         s.conjugate_pairs,
         s.refinement_policy,
         s.residual_target_ev,
+        s.resident_limit,
     )
 
 

@@ -19,8 +19,14 @@ With the literal recipe the CISS windows would hold hundreds to thousands of
 states, far beyond a 128-column search space; the gap half-widths below were
 chosen by measuring the in-window count with an FP64 dense eigensolver or
 Sylvester inertia: small2d 40-41 states in [-2.02, 2.02] eV, stress2d 64 in
-[-1.52, 1.52] eV, bulk3d 66-67 in [-1.02, 1.02] eV.  Every spec records its
-seed and knob in the bundle provenance.
+[-1.52, 1.52] eV, bulk3d 66-67 in [-1.02, 1.02] eV.  The slabs are too large
+for a dense or inertia count on the host; their gaps were chosen with the GPU
+CISS itself (block 64, moment 4, two filter passes) at Gamma, keeping the
+smallest gap whose count stays well below the search dimension with an FP64
+eigenpair residual below 1e-4 eV: slab52k 68 states in [-0.82, 0.82] eV
+(gap 2.75), slab26k 53 states in [-2.02, 2.02] eV (gap 4.5), the widest
+serving window.  Every spec records its seed and knob in the bundle
+provenance.
 
 The precision-stress bundle (stress2d) clusters its onsite levels on 96
 centres with 1e-3 eV spread and scales the off-site overlap so that the
@@ -103,13 +109,13 @@ SPECS: dict[str, SynthSpec] = {
         "bulk3d", (10, 10, 10), 2.4, (24.0, 24.0, 24.0), (True, True, True), 1002, gap=2.85
     ),
     "slab52k": SynthSpec(
-        "slab52k", (20, 20, 10), 2.4, (48.0, 48.0, 44.0), (True, True, False), 1003, gap=7.0
+        "slab52k", (20, 20, 10), 2.4, (48.0, 48.0, 44.0), (True, True, False), 1003, gap=2.75
     ),
     "slab26k_a": SynthSpec(
-        "slab26k_a", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2001, gap=7.0
+        "slab26k_a", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2001, gap=4.5
     ),
     "slab26k_b": SynthSpec(
-        "slab26k_b", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2002, gap=7.0
+        "slab26k_b", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2002, gap=4.5
     ),
 }
 

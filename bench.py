@@ -127,9 +127,10 @@ def serving_bench(args) -> int:
         log(f"batch {entry['bundle']}: {entry['k_points']} k from {entry['jobs']} job(s), "
             f"{entry['end'] - entry['start']:.2f} s" + (f", error {entry['error'][:80]}" if entry["error"] else ""))
 
-    with BandServer(k_batch=args.k_batch, on_batch=progress) as server:
+    kb = args.k_batch or 4
+    with BandServer(k_batch=kb, on_batch=progress) as server:
         tune_for = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision=args.precision,
-                                k_batch=args.k_batch)
+                                k_batch=kb)
         for name, b in bundles.items():
             server.register(name, b, autotune=None if args.no_autotune else tune_for)
             log(f"registered {name}")
@@ -190,7 +191,7 @@ def serving_bench(args) -> int:
             f"{{20,61,121}} x {args.kscale}, half-width U[0.5,2] eV, block {{32,64}}",
             "precision": args.precision,
             "refinement_policy": args.refinement_policy,
-            "k_batch": args.k_batch,
+            "k_batch": kb,
         },
         "data": "synthetic (seeded generator)",
     }
@@ -204,7 +205,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bulk3d")
-    ap.add_argument("--k-batch", type=int, default=4)
+    ap.add_argument("--k-batch", type=int, default=None, help="k points per batch (default: per workload)")
     ap.add_argument("--plain-kpoints", type=int, default=2)
     ap.add_argument("--precision", default="bf16x3_ir")
     ap.add_argument("--ir-steps", type=int, default=2)
@@ -215,7 +216,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serving", action="store_true", help="mixed request load through the serving loop")
-    ap.add_argument("--serving-bundles", default="small2d,stress2d,bulk3d")
+    ap.add_argument("--serving-bundles", default="small2d,stress2d,bulk3d,slab26k_a,slab26k_b")
     ap.add_argument("--jobs", type=int, default=12)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--kscale", type=float, default=0.1)
@@ -236,9 +237,16 @@ def main() -> int:
     from aospectrum_amd.data import EnergyInterval
     from aospectrum_amd.sparse import NativeBundle
 
-    windows = {"bulk3d": (-1.0, 1.0), "small2d": (-2.0, 2.0), "stress2d": (-1.5, 1.5)}
+    # window (eV), block size, k points per batch, points per path segment
+    workloads = {
+        "bulk3d": ((-1.0, 1.0), 32, 4, 40),
+        "small2d": ((-2.0, 2.0), 32, 4, 40),
+        "stress2d": ((-1.5, 1.5), 32, 4, 20),
+        "slab52k": ((-0.8, 0.8), 64, 1, 20),
+        "slab26k_a": ((-2.0, 2.0), 32, 2, 20),
+    }
+    (lo, hi), block, k_default, seg = workloads[args.workload]
     spec = synth.SPECS[args.workload]
-    lo, hi = windows.get(args.workload, (-1.0, 1.0))
     interval = EnergyInterval(lo - 0.02, hi + 0.02)
     t0 = time.perf_counter()
     bundle = synth.generate(spec)
@@ -252,10 +260,12 @@ def main() -> int:
         f"symbolic+upload {setup_s:.1f} s: nnz={info['nnz']} tiles={info['n_factor_tiles']} "
         f"flops/factor={info['factor_flops']:.3e}"
     )
-    path = ygxm_path(40)
-    K = args.k_batch
+    path = ygxm_path(seg)
+    K = args.k_batch or k_default
     tune = None
+    peak_bytes = 0
     mixed = CissSettings(
+        block_size=block,
         contour_batch_size=16,
         k_batch=K,
         conjugate_pairs=True,
@@ -267,7 +277,7 @@ def main() -> int:
     # plain: the reference's configuration -- one shift per factorisation
     # launch, one k point at a time, FP32 LU with two refinement steps (the
     # reference's cuDSS call sets IR_N_STEPS = 2)
-    plain = CissSettings(precision="fp32_ir", refinement_steps=2, refinement_passes=args.passes)
+    plain = CissSettings(block_size=block, precision="fp32_ir", refinement_steps=2, refinement_passes=args.passes)
 
     def kslice(step):
         # timed steps walk the path from its start; warmup steps (negative)
@@ -299,6 +309,7 @@ def main() -> int:
         t = time.perf_counter()
         res, binfo = band_solve(native, mixed, interval, kpoints=kp)
         lat.append(time.perf_counter() - t)
+        peak_bytes = max(peak_bytes, binfo["peak_device_bytes"])
         refined += [idx[i] for i in binfo["refined"]]
         for i, r in zip(idx, res):
             results[i] = r
@@ -314,6 +325,10 @@ def main() -> int:
     plain_info = {}
     if not args.no_plain and args.plain_kpoints > 0:
         pidx = sorted(results)[: args.plain_kpoints]
+        if bundle.n_orbitals > 20000:
+            # one line per shift group from the library: a plain k point of a
+            # large slab factors each shift again in every filter pass
+            os.environ["AOSX_PROGRESS"] = "1"
         native.kernel_stats(True)
         # plain warmup on a k point outside the timed set (captures its graphs)
         band_solve(native, plain, interval, kpoints=path.fractional_points[path.n_points - 1 :])
@@ -323,6 +338,7 @@ def main() -> int:
         pdt = time.perf_counter() - t
         pks = native.kernel_stats(False)
         plain_rate = len(pidx) / pdt
+        os.environ.pop("AOSX_PROGRESS", None)
         mismatch = [i for i, r in zip(pidx, pres) if r.energies_ev.size != results[i].energies_ev.size]
         dmax = max(
             (float(np.max(np.abs(r.energies_ev - results[i].energies_ev))) for i, r in zip(pidx, pres)
@@ -372,9 +388,9 @@ def main() -> int:
         "data": "synthetic (seeded generator, no datasets)",
         "config": {
             "workload": f"{spec.name}: n={bundle.n_orbitals}, {bundle.structure.n_atoms} atoms, "
-            f"nnz={info['nnz']}, 121-pt YGXM path, window [{lo},{hi}] eV",
+            f"nnz={info['nnz']}, {path.n_points}-pt YGXM path, window [{lo},{hi}] eV",
             "integration_points": 32,
-            "block_size": 32,
+            "block_size": block,
             "moment_size": 4,
             "k_batch": K,
             "precision": args.precision,
@@ -395,7 +411,11 @@ def main() -> int:
         "refined_kpoints": refined,
         "golden_counts": {str(i): {"oracle": c, "gpu": g} for i, (c, g) in golden.items()},
         "factor_tiles": info["n_factor_tiles"],
+        "factor_fill_entries": info["n_factor_tiles"] * 64 * 64,
+        "atom_fill_entries": info["atom_fill_entries"],
+        "factor_bytes_per_matrix": info["factor_bytes"],
         "factor_flops_per_matrix": info["factor_flops"],
+        "peak_device_bytes": peak_bytes,
         "roofline": {
             "bound": "mfma",
             "kernel": ("k_schur_quad_split" if args.precision.startswith("bf16") else "k_schur_group")

@@ -248,7 +248,10 @@ typedef struct {
     int32_t refinement_passes;     /* extra filter passes on the Ritz block        */
     int32_t k_batch;               /* k points factored together                   */
     int32_t conjugate_pairs;       /* 1: solve z_{N-1-j} = conj(z_j) with A^H      */
-    int32_t reserved;
+    int32_t resident_limit;        /* filter passes keep a batch's factorisations  */
+                                   /* resident when they fit in device memory and  */
+                                   /* number <= this (0: no limit); otherwise each */
+                                   /* pass factors every shift group again         */
     /* outputs */
     float *energies_ev;            /* [n_kpoints][capacity], ascending per k       */
     int32_t energy_capacity;       /* >= block_size * moment_size                  */

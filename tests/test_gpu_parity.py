@@ -241,6 +241,54 @@ def test_autotune_picks_a_timed_group_and_keeps_results(small2d, small2d_pattern
         nb.close()
 
 
+def test_streamed_factorisations_match_resident(native_small):
+    """With at most one factorisation allowed resident, every filter pass
+    factors each shift group again; the result is the resident one's."""
+    from dataclasses import replace
+
+    kpts = np.array([[0.13, 0.37, 0.0], [0.0, 0.0, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    resident = replace(MIXED_ON_DEMAND, refinement_policy="fixed", refinement_steps=1)
+    streamed = replace(resident, resident_limit=1)
+    say("band solve resident vs streamed")
+    a, ia = band_solve(native_small, resident, interval, kpoints=kpts)
+    b, ib = band_solve(native_small, streamed, interval, kpoints=kpts)
+    # moment pass plus two filter passes, each factoring all 2 x 16 shifts
+    assert ib["factorizations"] == 3 * ia["factorizations"]
+    for x, y in zip(a, b):
+        assert x.energies_ev.size == y.energies_ev.size
+        assert np.max(np.abs(x.energies_ev - y.energies_ev), initial=0.0) <= 1e-6
+        assert y.maximum_residual_ev <= 1e-4
+
+
+def test_slab26k_precisions_agree():
+    """26,000-orbital slab (too large for the dense oracle): the BF16x3 path
+    with refinement on demand and the FP32 path with two fixed refinement
+    steps find the same in-window states, each with an FP64 eigenpair
+    residual below 1e-4 eV."""
+    bundle = synth.generate(synth.SPECS["slab26k_a"])
+    nb = NativeBundle(bundle, device=0)
+    try:
+        k = np.array([[0.25, 0.0, 0.0]])
+        interval = EnergyInterval(-2.02, 2.02)  # the calibrated window (about 50 states)
+        base = dict(contour_batch_size=16, conjugate_pairs=True, refinement_passes=2)
+        runs = {}
+        for label, extra in (("bf16x3", dict(precision="bf16x3_ir", refinement_steps=2,
+                                              refinement_policy="on_demand")),
+                             ("fp32", dict(precision="fp32_ir", refinement_steps=2))):
+            say("slab26k", label)
+            (r,), info = band_solve(nb, CissSettings(**base, **extra), interval, kpoints=k)
+            say(f"  count={r.energies_ev.size} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e} "
+                f"peak={info['peak_device_bytes'] / 1e9:.1f} GB")
+            assert r.complete and r.maximum_residual_ev <= 1e-4
+            runs[label] = r
+        a, b = runs["bf16x3"], runs["fp32"]
+        assert a.energies_ev.size == b.energies_ev.size > 0
+        assert np.max(np.abs(a.energies_ev - b.energies_ev)) < 1e-4
+    finally:
+        nb.close()
+
+
 def test_band_solve_matches_oracle_ciss(small2d, small2d_pattern, native_small):
     """Single-pass plain CISS: same energies and rank as the CPU restatement
     of the reference algorithm (same probes, same settings)."""
