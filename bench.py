@@ -158,6 +158,7 @@ def serving_bench(args) -> int:
 
         natives = {name: NativeBundle(b, device=0) for name, b in bundles.items()}
         done = {r.job_id for r in results}
+        os.environ["AOSX_PROGRESS"] = "1"  # slab jobs run minutes per k point in the plain loop
         t1 = time.perf_counter()
         nk = 0
         for _, job in jobs:
@@ -169,6 +170,7 @@ def serving_bench(args) -> int:
             nk += job.kpoints.shape[0]
             log(f"plain {job.job_id} done")
         seq_rate = nk / (time.perf_counter() - t1)
+        os.environ.pop("AOSX_PROGRESS", None)
         for nb in natives.values():
             nb.close()
         log(f"sequential plain loop: {seq_rate:.3f} k-points/s")
