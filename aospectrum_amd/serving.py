@@ -148,9 +148,16 @@ class BandServer:
         ``autotune`` the bundle's step-group size is tuned for batches of
         those settings (``k_batch`` k points) before it takes jobs."""
         native = NativeBundle(bundle, device=self.device)
-        if autotune is not None:
-            native.autotune(autotune.factorizations_per_launch, autotune.precision)
         res = _Resident(key, native)
+        if autotune is not None:
+            # tune for the largest batch of these settings the budget admits,
+            # then give the tuning workspace back
+            per_k = autotune.factorizations_per_launch // autotune.k_batch
+            k = autotune.k_batch
+            while k > 1 and self._estimate(native, autotune, k) > self.budget:
+                k -= 1
+            native.autotune(k * per_k, autotune.precision)
+            native.trim()
         with self._lock:
             if key in self._bundles:
                 native.close()
@@ -162,14 +169,21 @@ class BandServer:
 
     def estimate_bytes(self, key: str, settings: CissSettings, n_k: int) -> int:
         """Device bytes one batch of ``n_k`` k points needs beyond residency."""
-        info = self._bundles[key].native.info
+        return self._estimate(self._bundles[key].native, settings, n_k)
+
+    @staticmethod
+    def _estimate(native: NativeBundle, settings: CissSettings, n_k: int) -> int:
+        info = native.info
         n = int(info["n"])
         nfac = settings.integration_points // (2 if settings.conjugate_pairs else 1)
         slots = n_k * (nfac if settings.refinement_passes else min(settings.contour_batch_size, nfac))
         width = settings.block_size * settings.moment_size + 2 * settings.block_size
         vectors = n_k * n * width * 8 * 12 + slots * n * width * 8 * 3
         matrices = n_k * int(info["nnz"]) * 16
-        return int(slots * int(info["factor_bytes"]) + vectors + matrices)
+        # BF16 precisions: split copies of one step group's panel tiles per
+        # factorisation (at most 2 x 32 steps x n_diag tiles of 32 KiB)
+        split = 2 * 32 * int(info["n_diagonal_tiles"]) * 32768 if settings.precision.startswith("bf16") else 0
+        return int(slots * (int(info["factor_bytes"]) + split) + vectors + matrices)
 
     # -------------------------------------------------------------- jobs
     def submit(self, job: BandJob) -> Future:
