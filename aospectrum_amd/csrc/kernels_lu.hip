@@ -583,10 +583,17 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
             if (more) {
                 t = g0 + __builtin_ctz(mask);
                 mask &= mask - 1;
-                ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
-                rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
             }
+            // the next step's tiles (the current step's again after the
+            // last), their loads spread over this step's products
+            ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
+            rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
             mma_split<THREE>(As, Bs, cre, cim, wave, lane);
+#pragma unroll
+            for (int g = 0; g < 16; ++g) {
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);              // 1 VMEM read
+                __builtin_amdgcn_sched_group_barrier(0x008, THREE ? 3 : 1, 0);  // MFMA
+            }
             if (!more) break;
         }
     }
@@ -749,14 +756,27 @@ __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
                 all &= all - 1;
             }
             offsets(t, o);
-            ra0 = fetch_split(sp + o[0], tid);
-            ra1 = fetch_split(sp + o[1], tid);
-            rb0 = fetch_split(sp + o[2], tid);
-            rb1 = fetch_split(sp + o[3], tid);
             if (mine & (1u << (tcur - g0))) {
+                // the next step's loads spread over this step's products: one
+                // load per few MFMAs, so the products never wait behind a
+                // full memory queue
+                ra0 = fetch_split(sp + o[0], tid);
+                ra1 = fetch_split(sp + o[1], tid);
+                rb0 = fetch_split(sp + o[2], tid);
+                rb1 = fetch_split(sp + o[3], tid);
                 const uint16_t *As = ia ? A1 : A0, *Bs = jb ? B1 : B0;
 #pragma unroll
                 for (int k0 = 0; k0 < TILE; k0 += 16) mma_split_quad_chunk(As, Bs, cre, cim, lane, k0, THREE);
+#pragma unroll
+                for (int g = 0; g < 32; ++g) {
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);          // 1 VMEM read
+                    __builtin_amdgcn_sched_group_barrier(0x008, THREE ? 6 : 2, 0);  // MFMA
+                }
+            } else {
+                ra0 = fetch_split(sp + o[0], tid);
+                ra1 = fetch_split(sp + o[1], tid);
+                rb0 = fetch_split(sp + o[2], tid);
+                rb1 = fetch_split(sp + o[3], tid);
             }
             if (!more) break;
         }

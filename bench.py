@@ -434,6 +434,9 @@ def main() -> int:
             "mfma_issued_tflops": achieved * MFMA_PRODUCTS[args.precision],
         },
         "kernel_ms": {k: v for k, v in ks.items() if k.endswith("_ms")},
+        # streaming stages in achieved algorithmic GB/s (HBM peak 8,000)
+        "assembly_gbps": ks["assemble_bytes"] / max(ks["assemble_ms"], 1e-9) / 1e6,
+        "solve_gbps": ks["solve_bytes"] / max(ks["solve_ms"], 1e-9) / 1e6,
         "cpu_baseline": cpu,
         "setup_seconds": {"generate": gen_s, "symbolic_and_upload": setup_s},
         "autotune": tune,
