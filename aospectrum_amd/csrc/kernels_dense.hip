@@ -337,13 +337,25 @@ __global__ void __launch_bounds__(1024) k_jacobi(c64 *__restrict__ A_all, c64 *_
     const int npairs = rr / 2;
     int sweep = 0;
     __syncthreads();
+    // element loops walk (row, column) or (row, pair) indices incrementally:
+    // thread tid starts at flat index tid and advances by nthr
+    auto start = [&](int width, int &a, int &b, int &da, int &db) {
+        a = tid / width;
+        b = tid % width;
+        da = nthr / width;
+        db = nthr % width;
+    };
     for (; sweep < max_sweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
-        for (int e = tid; e < r * r; e += nthr) {
-            const int i = e / r, j = e % r;
-            const double v = cabs2(A[(int64_t)i * ld + j]);
-            tot += v;
-            if (i != j) off += v;
+        {
+            int i, j, di, dj;
+            start(r, i, j, di, dj);
+            for (; i < r; i += di, j += dj) {
+                if (j >= r) { j -= r; ++i; if (i >= r) break; }
+                const double v = cabs2(A[(int64_t)i * ld + j]);
+                tot += v;
+                if (i != j) off += v;
+            }
         }
         red[tid] = off;
         red2[tid] = tot;
@@ -381,8 +393,10 @@ __global__ void __launch_bounds__(1024) k_jacobi(c64 *__restrict__ A_all, c64 *_
             }
             __syncthreads();
             // columns: A <- A G, V <- V G
-            for (int e = tid; e < r * npairs; e += nthr) {
-                const int i = e / npairs, pi = e % npairs;
+            int i, pi, di, dpi;
+            start(npairs, i, pi, di, dpi);
+            for (; i < r; i += di, pi += dpi) {
+                if (pi >= npairs) { pi -= npairs; ++i; if (i >= r) break; }
                 const int p = rp[pi], q = rq[pi];
                 if (p == q || rs[pi] == 0.0) continue;
                 const double c = rc[pi], s = rs[pi];
@@ -398,12 +412,14 @@ __global__ void __launch_bounds__(1024) k_jacobi(c64 *__restrict__ A_all, c64 *_
             }
             __syncthreads();
             // rows: A <- G^H A
-            for (int e = tid; e < r * npairs; e += nthr) {
-                const int pi = e / r, j = e % r;
-                const int p = rp[pi], q = rq[pi];
-                if (p == q || rs[pi] == 0.0) continue;
-                const double c = rc[pi], s = rs[pi];
-                const c64 ep = {rpr[pi], rpi[pi]};  // e^{+i phi}
+            int qi, j, dqi, dj;
+            start(r, qi, j, dqi, dj);
+            for (; qi < npairs; qi += dqi, j += dj) {
+                if (j >= r) { j -= r; ++qi; if (qi >= npairs) break; }
+                const int p = rp[qi], q = rq[qi];
+                if (p == q || rs[qi] == 0.0) continue;
+                const double c = rc[qi], s = rs[qi];
+                const c64 ep = {rpr[qi], rpi[qi]};  // e^{+i phi}
                 c64 *ap = A + (int64_t)p * ld + j, *aq = A + (int64_t)q * ld + j;
                 const c64 x = *ap, y = cmul(ep, *aq);
                 *ap = {c * x.re - s * y.re, c * x.im - s * y.im};

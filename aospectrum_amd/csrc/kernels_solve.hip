@@ -53,9 +53,47 @@ __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64
 // The same with both operands split into BF16 hi + lo parts (three MFMA
 // products, FP32 accumulation; refinement on FP64 residuals removes the
 // difference): the BF16 precisions.
+// Split RHS slots: the finished Y_k of step k is also kept, split and
+// transposed (planes [re_hi, re_lo, im_hi, im_lo] x [column][row]), in slot
+// k % YSLOTS of the matrix's slot area, the B operand layout of the update
+// products; a step group spans at most 32 steps, so its slots are distinct.
+constexpr int YSLOTS = 64;
+
+__device__ __forceinline__ uint16_t *yslot(uint16_t *ys, int m, int k, int nchunk, int c) {
+    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * SPLIT_TILE;
+}
+__device__ __forceinline__ const uint16_t *yslot(const uint16_t *ys, int m, int k, int nchunk, int c) {
+    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * SPLIT_TILE;
+}
+
+// Writes the wave's 32 x 32 accumulator block as split, transposed planes:
+// lane l holds column c = l % 32 (of the block), rows 8q + 4(l/32) + 0..3.
+__device__ __forceinline__ void store_acc_split_t(uint16_t *__restrict__ dst, const floatx16 &cre,
+                                                  const floatx16 &cim, int wave, int lane) {
+    const int c = (wave & 1) * 32 + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;
+        uint16_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            split_pair(cre[4 * q + e], rh[e], rl[e]);
+            split_pair(cim[4 * q + e], ih[e], il[e]);
+        }
+        const int off = c * TILE + row;
+        auto pack = [](const uint16_t *v) {
+            return make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
+        };
+        *reinterpret_cast<uint2 *>(dst + off) = pack(rh);
+        *reinterpret_cast<uint2 *>(dst + TILE2 + off) = pack(rl);
+        *reinterpret_cast<uint2 *>(dst + 2 * TILE2 + off) = pack(ih);
+        *reinterpret_cast<uint2 *>(dst + 3 * TILE2 + off) = pack(il);
+    }
+}
+
 __global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                             const float *__restrict__ dinv, int nt, int k, int which,
-                                                            int ctrans) {
+                                                            int ctrans, uint16_t *__restrict__ ysplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int m = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
@@ -68,6 +106,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__
         floatx16 cre = {}, cim = {};
         mma_split<true>(As, Bs, cre, cim, wave, lane);
         store_acc(yk, cre, cim, wave, lane);
+        store_acc_split_t(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
     }
 }
 
@@ -104,7 +143,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict
                                                               int nchunk, const float *__restrict__ tiles,
                                                               int64_t mat_stride, const int32_t *__restrict__ tile_id,
                                                               int nt, int k, const int32_t *__restrict__ list,
-                                                              int ctrans) {
+                                                              int ctrans, const uint16_t *__restrict__ ysplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int i = list[blockIdx.x];
@@ -112,10 +151,10 @@ __global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict
     const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
     stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
     for (int c = 0; c < nchunk; ++c) {
-        const float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
         float *yi = y + rhs_block(y_mat_stride, m, i, nchunk, c);
+        const u32x32 yv = fetch_split(yslot(ysplit, m, k, nchunk, c), tid);
         __syncthreads();
-        stage_split(yk, Bs, true, false, false, tid);
+        put_split(Bs, yv, tid);
         floatx16 cre, cim;
         load_acc(yi, cre, cim, wave, lane);
         __syncthreads();
@@ -138,7 +177,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
                                                        const float *__restrict__ tiles, int64_t mat_stride,
                                                        const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
                                                        const int32_t *__restrict__ rows, int nrows, int nmat,
-                                                       int ctrans) {
+                                                       int ctrans, const uint16_t *__restrict__ ysplit) {
     // FP32 operands (Are/Aim/Bre/Bim) or split operands (As/Bs) share one
     // LDS area.
     __shared__ __attribute__((aligned(16))) float lds[SPLIT ? (8 * SPL) / 2 : 2 * TILE * LDA + 2 * TILE2];
@@ -170,9 +209,8 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
         for (int c = 0; c < NC; ++c) {
             if (c >= nc) break;
             if (c > 0) __syncthreads();
-            const float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c);
-            if (SPLIT) stage_split(yk, Bs, true, false, false, tid);
-            else stage_b(yk, Bre, Bim, tid);
+            if (SPLIT) put_split(Bs, fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid), tid);
+            else stage_b(y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c), Bre, Bim, tid);
             __syncthreads();
             if (SPLIT) mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
             else mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
@@ -184,9 +222,9 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
 }
 
 template __global__ void k_solve_trail<false, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                 int, int, const int32_t *, int, int, int);
+                                                 int, int, const int32_t *, int, int, int, const uint16_t *);
 template __global__ void k_solve_trail<true, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                int, int, const int32_t *, int, int, int);
+                                                int, int, const int32_t *, int, int, int, const uint16_t *);
 
 // Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
 // (row-major n x w_src complex64); columns >= w_src and padded rows are zero.

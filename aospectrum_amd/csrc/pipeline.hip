@@ -618,13 +618,32 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
 // y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
 void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision) {
     const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
-    auto diag = split ? k_solve_diag_split : k_solve_diag;
-    auto update = split ? k_solve_update_split : k_solve_update;
     float *y = reinterpret_cast<float *>(y2);
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
     auto trail = split ? k_solve_trail<true, 2> : k_solve_trail<false, 2>;
+    // split RHS slots (BF16 precisions): finished Y_k blocks, split once
+    uint16_t *ysplit = split ? buf<uint16_t>("s_ysplit", (size_t)nmat * YSLOTS * nchunk * SPLIT_TILE) : nullptr;
+    // Y_k <- D_k Y_k (which: 0 inv(L_kk), 1 inv(U_kk)), and the split copy
+    auto diag_launch = [&](int k, int which) {
+        if (split)
+            k_solve_diag_split<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ctrans ? 1 : 0,
+                                                         ysplit);
+        else
+            k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+    };
+    // Y_i -= T_ik Y_k for the `count` rows listed at `rows_dev`
+    auto update_launch = [&](int k, const int32_t *rows_dev, int count) {
+        if (split)
+            k_solve_update_split<<<dim3(count, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
+                                                                         nt, k, rows_dev, ctrans ? 1 : 0, ysplit);
+        else
+            k_solve_update<<<dim3(count, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt,
+                                                                   k, rows_dev, ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+    };
     const int ngrp = (nchunk + 1) / 2;
     const int ng = (int)grp_begin.size() - 1;
     auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
@@ -632,37 +651,31 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         const int64_t items = nrows * nmat * ngrp;
         trail<<<(unsigned)(8 * ((items + 7) / 8)), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
                                                                     nt, g0, g1, rows, (int)nrows, nmat,
-                                                                    ctrans ? 1 : 0);
+                                                                    ctrans ? 1 : 0, ysplit);
         AOSX_LAUNCHED();
     };
     // Step groups pay off when a launch covers many matrices; with a few, the
     // per-step sweep (more, shorter launches) keeps more of the GPU busy.
     const bool grouped = nmat >= 8;
-    // algorithmic bytes: every stored tile is read once per sweep pair,
-    // every RHS block is read and written once per tile it meets
+    // algorithmic bytes: every stored tile is read once per forward +
+    // backward pair, every RHS block read and written once per direction
     const double bytes = (double)nmat * ((double)ts.n_tiles * TILE_FLOATS * 4.0 +
-                                         2.0 * (double)ts.next_idx.size() * nchunk * TILE_FLOATS * 4.0 * 3.0);
+                                         4.0 * (double)nt * nchunk * TILE_FLOATS * 4.0);
     auto launches = [&]() {
         const int r = range_begin(RANGE_SOLVE);
         if (!grouped) {
             for (int k = 0; k < nt; ++k) {
-                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
-                AOSX_LAUNCHED();
+                diag_launch(k, ctrans ? 1 : 0);
                 const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
                 if (s) {
-                    update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
-                    AOSX_LAUNCHED();
+                    update_launch(k, d_next + ts.next_ptr[k], s);
                 }
             }
             for (int k = nt - 1; k >= 0; --k) {
-                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
-                AOSX_LAUNCHED();
+                diag_launch(k, ctrans ? 0 : 1);
                 const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
                 if (s) {
-                    update<<<dim3(s, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_prev + ts.prev_ptr[k], ctrans ? 1 : 0);
-                    AOSX_LAUNCHED();
+                    update_launch(k, d_prev + ts.prev_ptr[k], s);
                 }
             }
         }
@@ -671,16 +684,13 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         for (int g = 0; grouped && g < ng; ++g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
             for (int k = g0; k < g1; ++k) {
-                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 1 : 0, ctrans ? 1 : 0);
-                AOSX_LAUNCHED();
+                diag_launch(k, ctrans ? 1 : 0);
                 const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
                 const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
                 int p = 0;
                 while (p < s && nx[p] < g1) ++p;
                 if (p) {
-                    update<<<dim3(p, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                              d_next + ts.next_ptr[k], ctrans ? 1 : 0);
-                    AOSX_LAUNCHED();
+                    update_launch(k, d_next + ts.next_ptr[k], p);
                 }
             }
             trailing(g0, g1, d_fwd_rows + fwd_row_ptr[g], fwd_row_ptr[g + 1] - fwd_row_ptr[g]);
@@ -689,23 +699,21 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         for (int g = ng - 1; grouped && g >= 0; --g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
             for (int k = g1 - 1; k >= g0; --k) {
-                diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, ctrans ? 0 : 1, ctrans ? 1 : 0);
-                AOSX_LAUNCHED();
+                diag_launch(k, ctrans ? 0 : 1);
                 const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
                 const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
                 int a = s;
                 while (a > 0 && pv[a - 1] >= g0) --a;
                 if (a < s) {
-                    update<<<dim3(s - a, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, k,
-                                                                  d_prev + ts.prev_ptr[k] + a, ctrans ? 1 : 0);
-                    AOSX_LAUNCHED();
+                    update_launch(k, d_prev + ts.prev_ptr[k] + a, s - a);
                 }
             }
             trailing(g0, g1, d_bwd_rows + bwd_row_ptr[g], bwd_row_ptr[g + 1] - bwd_row_ptr[g]);
         }
         range_end(r, bytes);
     };
-    run_sequence({1, nmat, W, (int64_t)y, (int64_t)tiles, (int64_t)dinv, ctrans ? 1 : 0, precision}, launches);
+    run_sequence({1, nmat, W, (int64_t)y, (int64_t)tiles, (int64_t)dinv, ctrans ? 1 : 0, precision, (int64_t)ysplit},
+                 launches);
 }
 
 void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
