@@ -21,9 +21,10 @@
 // steps they apply, so every target is read and written once per group.
 // The complex products use four real FP32 MFMA products (v_mfma_f32_32x32x2_f32),
 // i.e. 8 real flops per complex multiply-add.  In the BF16 precisions the
-// panels also write their tiles split into hi + lo BF16 parts, which the Schur
-// updates feed to v_mfma_f32_32x32x16_bf16 (three products, FP32
-// accumulation); the stored factor stays FP32.
+// panel and Schur products split every operand into hi + lo BF16 parts for
+// v_mfma_f32_32x32x16_bf16 (three products, FP32 accumulation), the panels
+// also write their tiles in that split form for the group's updates, and the
+// stored factor stays FP32.
 #include "device_common.h"
 
 namespace aosx {
@@ -349,14 +350,12 @@ __device__ __forceinline__ void write_split(const float *Tre, const float *Tim, 
 }
 
 // Panel step k: block x < s updates L(next[x], k) <- L * inv(U_kk); block
-// x >= s updates U(k, next[x - s]) <- inv(L_kk) * U.  With `split` set, the
-// result is also written as split tile slot0 + x of the matrix's split area
-// (split_stride split tiles per matrix).
+// x >= s updates U(k, next[x - s]) <- inv(L_kk) * U.  FP32 MFMA (the BF16
+// precisions use k_panel_split).
 __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_t mat_stride,
                                               const float *__restrict__ dinv, int nt, int k,
                                               const int32_t *__restrict__ next, int s,
-                                              const int32_t *__restrict__ tile_id, uint16_t *__restrict__ split,
-                                              int64_t split_stride, int slot0) {
+                                              const int32_t *__restrict__ tile_id) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -376,26 +375,14 @@ __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_
     floatx16 cre = {}, cim = {};
     mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
     store_acc(t, cre, cim, wave, lane);
-    if (split != nullptr) {
-        __syncthreads();  // every wave is done reading the operands
-        const int r0 = (wave >> 1) * 32 + 4 * (lane >> 5), c = (wave & 1) * 32 + (lane & 31);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = r0 + 8 * (r >> 2) + (r & 3);
-            Are[row * LDA + c] = cre[r];
-            Aim[row * LDA + c] = cim[r];
-        }
-        __syncthreads();
-        uint16_t *dst = split + ((int64_t)m * split_stride + slot0 + blockIdx.x) * SPLIT_TILE;
-        write_split(Are, Aim, dst, lower, tid);
-    }
 }
 
 // Multi-step update: A(i,j) -= sum_t L(i,t) U(t,j) over the steps t in
 // [g0, g1) where both tiles exist, for every pair (i, j) of the list (a
 // group's trailing targets, or the row/column targets of one step with the
 // group's earlier steps).  The target stays in the MFMA accumulators across
-// the steps, so it is read and written once rather than once per step.  The grid is 8 * ceil(npairs * nmat / 8) blocks in one
+// the steps, so it is read and written once rather than once per step.  The
+// grid is 8 * ceil(npairs * nmat / 8) blocks in one
 // dimension; consecutive blocks go round-robin to the 8 XCDs, so block b works
 // on item (b % 8) * per + b / 8 and each XCD (with its own L2) takes one
 // contiguous run of the (matrix, pair) items, whose pairs share L rows and U
@@ -519,6 +506,48 @@ __device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_
         *reinterpret_cast<uint2 *>(S + 3 * SPL + off) =
             make_uint2(il[0] | ((uint32_t)il[1] << 16), il[2] | ((uint32_t)il[3] << 16));
     }
+}
+
+// Panel step k in the BF16 precisions: the products of k_panel with both
+// operands split into BF16 hi + lo parts (three MFMA products, FP32
+// accumulation, the accuracy of the Schur updates); writes the FP32 tile and
+// its split copy.
+__global__ void __launch_bounds__(256, 2) k_panel_split(float *__restrict__ tiles, int64_t mat_stride,
+                                                       const float *__restrict__ dinv, int nt, int k,
+                                                       const int32_t *__restrict__ next, int s,
+                                                       const int32_t *__restrict__ tile_id,
+                                                       uint16_t *__restrict__ split, int64_t split_stride,
+                                                       int slot0) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
+    const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool lower = (int)blockIdx.x < s;
+    const int other = next[lower ? blockIdx.x : blockIdx.x - s];
+    const int32_t id = lower ? tile_id[(int64_t)other * nt + k] : tile_id[(int64_t)k * nt + other];
+    float *t = tile_ptr(tiles, mat_stride, m, id);
+    const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
+    if (lower) {
+        stage_split(t, As, false, false, false, tid);               // L(i,k)
+        stage_split(d + TILE_FLOATS, Bs, true, false, false, tid);  // inv(U_kk)
+    } else {
+        stage_split(d, As, false, false, false, tid);               // inv(L_kk)
+        stage_split(t, Bs, true, false, false, tid);                // U(k,j)
+    }
+    __syncthreads();
+    floatx16 cre = {}, cim = {};
+    mma_split<true>(As, Bs, cre, cim, wave, lane);
+    store_acc(t, cre, cim, wave, lane);
+    __syncthreads();  // every wave is done reading the operands
+    float *Tre = reinterpret_cast<float *>(As), *Tim = Tre + TILE * LDA;  // 33 KiB of the 74 KiB
+    const int r0 = (wave >> 1) * 32 + 4 * (lane >> 5), c = (wave & 1) * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 8 * (r >> 2) + (r & 3);
+        Tre[row * LDA + c] = cre[r];
+        Tim[row * LDA + c] = cim[r];
+    }
+    __syncthreads();
+    uint16_t *dst = split + ((int64_t)m * split_stride + slot0 + blockIdx.x) * SPLIT_TILE;
+    write_split(Tre, Tim, dst, lower, tid);
 }
 
 // One split tile = 2048 chunks of 16 B; thread tid moves chunks q * 256 + tid.

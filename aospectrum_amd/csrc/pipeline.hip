@@ -581,8 +581,11 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
                 range_end(r, 0.0);
                 if (s == 0) continue;
                 r = range_begin(RANGE_PANEL);
-                k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id,
-                                                               split, split_tiles, slot0);
+                if (split_mode)
+                    k_panel_split<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s,
+                                                                         d_tile_id, split, split_tiles, slot0);
+                else
+                    k_panel<<<dim3(2 * s, nmat), 256, 0, stream>>>(tiles, mstride, dinv, nt, k, nx, s, d_tile_id);
                 AOSX_LAUNCHED();
                 range_end(r, 0.0);
             }
