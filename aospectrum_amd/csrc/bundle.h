@@ -141,6 +141,9 @@ struct Bundle {
     // the first call with a given key (sizes, precision, buffer addresses)
     // and replayed afterwards; AOSX_GRAPHS=0 runs them eagerly.
     bool use_graphs = true;
+    // trailing quad updates pipelined by K halves (AOSX_QUAD_PIPELINE=0: the
+    // step-at-a-time kernel)
+    bool quad_pipelined = true;
     std::map<std::vector<int64_t>, GraphEntry> graphs;
     std::vector<TimedRange> *ranges = nullptr;  // collecting ranges of the running sequence
     bool capturing = false;

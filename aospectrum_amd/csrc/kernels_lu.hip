@@ -303,8 +303,21 @@ __device__ __forceinline__ void store_acc(float *__restrict__ t, const floatx16 
 // four planes of 64 x 64 BF16 -- re_hi, re_lo, im_hi, im_lo, with
 // x ~ hi + lo, hi = bf16(x), lo = bf16(x - hi) (round to nearest even) --
 // holding -L(i,k) row-major ([row][col]) or U(k,j) transposed ([col][row]), so
-// that both MFMA operands read 8 consecutive k values per lane.
+// that both MFMA operands read 8 consecutive k values per lane.  The tile is
+// stored in MFMA operand order: 1 KiB blocks, one per (K half, plane, 32-row
+// half a, 16-k step within the half), each holding the 32 x 16 operand of one
+// v_mfma_f32_32x32x16_bf16 as lane l of the wave reads it -- row 32a + l % 32,
+// k values 8 (l / 32) .. + 7 -- in lane order.  A K half of all four planes is
+// 16 KiB of consecutive memory, and a wave reads an operand block from LDS
+// as one linear, bank-conflict-free 16-byte-per-lane read.
 constexpr int SPLIT_TILE = 4 * TILE2;  // BF16 elements per split tile (32 KiB)
+constexpr int SPLIT_HPLANE = TILE * 32;      // one plane of one K half (4 blocks)
+constexpr int SPLIT_HALF = 4 * SPLIT_HPLANE; // all planes of one K half (16 blocks)
+constexpr int SPLIT_BLOCK = 512;             // BF16 per operand block
+__device__ __forceinline__ int split_index(int plane, int row, int k) {
+    return (k >> 5) * SPLIT_HALF + plane * SPLIT_HPLANE + (row >> 5) * 1024 + ((k >> 4) & 1) * SPLIT_BLOCK +
+           ((k >> 3) & 1) * 256 + (row & 31) * 8 + (k & 7);
+}
 
 __device__ __forceinline__ void split_pair(float x, uint16_t &hi, uint16_t &lo) {
     const __bf16 h = (__bf16)x;
@@ -334,19 +347,18 @@ __device__ __forceinline__ void write_split(const float *Tre, const float *Tim, 
         split_pair(xr, rh[c], rl[c]);
         split_pair(xi, ih[c], il[c]);
     }
-    const int off = r * TILE + c0;
-    uint4 *d0 = reinterpret_cast<uint4 *>(dst + off);
-    uint4 *d1 = reinterpret_cast<uint4 *>(dst + TILE2 + off);
-    uint4 *d2 = reinterpret_cast<uint4 *>(dst + 2 * TILE2 + off);
-    uint4 *d3 = reinterpret_cast<uint4 *>(dst + 3 * TILE2 + off);
+    // k = c0 .. c0 + 7 and c0 + 8 .. c0 + 15: two 8-element runs of one block
+    const int off = split_index(0, r, c0), off8 = split_index(0, r, c0 + 8);
     auto pack = [](const uint16_t *v, int o) {
         return make_uint4(v[o] | ((uint32_t)v[o + 1] << 16), v[o + 2] | ((uint32_t)v[o + 3] << 16),
                           v[o + 4] | ((uint32_t)v[o + 5] << 16), v[o + 6] | ((uint32_t)v[o + 7] << 16));
     };
-    d0[0] = pack(rh, 0); d0[1] = pack(rh, 8);
-    d1[0] = pack(rl, 0); d1[1] = pack(rl, 8);
-    d2[0] = pack(ih, 0); d2[1] = pack(ih, 8);
-    d3[0] = pack(il, 0); d3[1] = pack(il, 8);
+    const uint16_t *planes[4] = {rh, rl, ih, il};
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) {
+        *reinterpret_cast<uint4 *>(dst + pl * SPLIT_HPLANE + off) = pack(planes[pl], 0);
+        *reinterpret_cast<uint4 *>(dst + pl * SPLIT_HPLANE + off8) = pack(planes[pl], 8);
+    }
 }
 
 // Panel step k: block x < s updates L(next[x], k) <- L * inv(U_kk); block
@@ -571,9 +583,10 @@ __device__ __forceinline__ u32x32 fetch_split(const uint16_t *__restrict__ src, 
 __device__ __forceinline__ void put_split(uint16_t *dst, const u32x32 r, int tid) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int e = q * 256 + tid;                 // chunk in the tile
-        const int plane = e >> 9, row = (e >> 3) & 63, c8 = e & 7;
-        *reinterpret_cast<uint4 *>(dst + plane * SPL + row * SLD + c8 * 8) =
+        const int e = q * 256 + tid;  // 8-element chunk of the split tile
+        const int plane = (e >> 8) & 3, row = ((e >> 7) & 1) * 32 + (e & 31);
+        const int k = (e >> 10) * 32 + ((e >> 6) & 1) * 16 + ((e >> 5) & 1) * 8;
+        *reinterpret_cast<uint4 *>(dst + plane * SPL + row * SLD + k) =
             make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
     }
 }
@@ -819,5 +832,179 @@ template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, in
 template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
                                                    const int32_t *, const uint16_t *, int64_t, const int32_t *,
                                                    const int32_t *);
+
+
+// Trailing update of a step group on 2 x 2 blocks of targets, pipelined by K
+// quarters (BF16 precisions): the same quads, step masks and products as
+// k_schur_quad_split, each accumulator in the same order, so the results
+// agree bit for bit.  Each step's four split tiles pass through LDS in four
+// quarters of 16 K columns, double-buffered (2 x 32 KiB), which leaves room
+// for two blocks per CU: while one block waits at its barrier or on its
+// target tiles, the other one's products keep the matrix cores busy.  The
+// quarters move from memory straight into LDS (buffer loads with the LDS
+// destination: no registers and no LDS write instructions), one quarter
+// ahead.  Split tiles are stored as operand blocks in lane order, so every
+// block is one 1 KiB load and one conflict-free 16-byte-per-lane LDS read.
+constexpr int QQ_BUF = 4 * 8 * SPLIT_BLOCK;  // four tiles x eight operand blocks (32 KiB)
+
+// wave w moves the 8 operand blocks (plane, row half) of K quarter q of tile w
+// into buffer `buf`
+__device__ __forceinline__ void qq_issue(uint16_t *buf, const uint16_t *__restrict__ sp, int4 slot, int q, int wave,
+                                         int lane) {
+    const int s = __builtin_amdgcn_readfirstlane(wave == 0 ? slot.x : wave == 1 ? slot.y : wave == 2 ? slot.z : slot.w);
+    // buffer (not flat) loads: the compiler then tracks them as memory loads
+    // into LDS only, and the LDS reads of the other buffer wait for none of them
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t *>(sp + (int64_t)s * SPLIT_TILE), 0, SPLIT_TILE * 2, 0x00020000);
+    uint16_t *d = buf + wave * 8 * SPLIT_BLOCK;
+    const int first = (q >> 1) * 16 + (q & 1);  // block of (plane 0, row half 0) in quarter q
+#pragma unroll
+    for (int j = 0; j < 8; ++j)  // j = 2 * plane + row half
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, reinterpret_cast<void *>(d + j * SPLIT_BLOCK), 16,
+                                                 ((first + 2 * j) * SPLIT_BLOCK + lane * 8) * 2, 0, 0, 0);
+}
+
+// products of one quarter buffer: A from tile ua, B from tile ub.  `keep` is
+// all ones, or zero when this wave has no update in the step: its A operands
+// are then zero and the products add nothing (instead of a branch, which would
+// keep the loads and the products of the other waves apart).  Each of the 8
+// accumulators takes its six products in the order of mma_split_quad_chunk;
+// the accumulators take turns, so consecutive MFMAs never depend on each other.
+__device__ __forceinline__ bf16x8 lds8_and(const uint16_t *p, uint32_t keep) {
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, lds8(p)) & (u32x4)(keep));
+}
+__device__ __forceinline__ void qq_mma(const uint16_t *buf, int ua, int ub, floatx16 (&cre)[4], floatx16 (&cim)[4],
+                                       int lane, bool three, uint32_t keep) {
+    auto blk = [&](int u, int pl, int a) { return buf + (u * 8 + pl * 2 + a) * SPLIT_BLOCK + lane * 8; };
+    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2], npih[2], npil[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        prh[h] = lds8_and(blk(ua, 0, h), keep);
+        pih[h] = lds8_and(blk(ua, 2, h), keep);
+        urh[h] = lds8(blk(ub, 0, h));
+        uih[h] = lds8(blk(ub, 2, h));
+        if (three) {
+            prl[h] = lds8_and(blk(ua, 1, h), keep);
+            pil[h] = lds8_and(blk(ua, 3, h), keep);
+            url[h] = lds8(blk(ub, 1, h));
+            uil[h] = lds8(blk(ub, 3, h));
+        }
+        npih[h] = neg8(pih[h]);
+        npil[h] = three ? neg8(pil[h]) : npih[h];
+    }
+    const int np = three ? 6 : 2;
+#pragma unroll
+    for (int p = 0; p < np; ++p)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int a = t >> 1, b = t & 1;
+            // re += Pr Ur - Pi Ui (hi hi, then hi lo, lo hi), im += Pr Ui + Pi Ur
+            const bf16x8 xr[6] = {prh[a], npih[a], prh[a], prl[a], npih[a], npil[a]};
+            const bf16x8 yr[6] = {urh[b], uih[b], url[b], urh[b], uil[b], uih[b]};
+            const bf16x8 xi[6] = {prh[a], pih[a], prh[a], prl[a], pih[a], pil[a]};
+            const bf16x8 yi[6] = {uih[b], urh[b], uil[b], uih[b], url[b], urh[b]};
+            cre[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xr[p], yr[p], cre[t], 0, 0, 0);
+            cim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xi[p], yi[p], cim[t], 0, 0, 0);
+        }
+}
+
+template <bool THREE>
+__global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ quads,
+    int nquads, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
+    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    // two buffers as two LDS objects: the compiler then knows that the reads
+    // of one do not wait for the direct loads into the other
+    __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
+    __shared__ __attribute__((aligned(1024))) uint16_t lds1[QQ_BUF];
+    __shared__ int4 slots[32];  // split slots of the four tiles of the i-th listed step
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = (int)(item / nquads), q = (int)(item % nquads);
+    const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
+    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
+    uint32_t rm[2] = {0u, 0u}, cm[2] = {0u, 0u};
+    for (int t = g0; t < g1; ++t) {
+        const int32_t *pp = ppos + (int64_t)t * nt;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (I[h] >= 0 && pp[I[h]] >= 0) rm[h] |= 1u << (t - g0);
+            if (J[h] >= 0 && pp[J[h]] >= 0) cm[h] |= 1u << (t - g0);
+        }
+    }
+    const int ia = wave >> 1, jb = wave & 1;
+    const uint32_t mine = rm[ia] & cm[jb];
+    const uint32_t all = (rm[0] & cm[0]) | (rm[0] & cm[1]) | (rm[1] & cm[0]) | (rm[1] & cm[1]);
+    const int n = __builtin_popcount(all);
+    // listed steps: the set bits of `all`, ascending; this wave's work as a
+    // mask over the list
+    uint32_t work = 0;
+    {
+        uint32_t rest = all;
+        for (int i = 0; rest; ++i) {
+            const int b = __builtin_ctz(rest);
+            rest &= rest - 1;
+            if (mine & (1u << b)) work |= 1u << i;
+            if (tid == i) {
+                // an absent tile takes a present one's slot: its half is
+                // loaded but read by no wave
+                const uint32_t bit = 1u << b;
+                const int32_t *pp = ppos + (int64_t)(g0 + b) * nt;
+                const int ss = step_s[g0 + b];
+                const bool any_c = (cm[0] | cm[1]) & bit, any_r = (rm[0] | rm[1]) & bit;
+                int s0 = (rm[0] & bit) && any_c ? pp[I[0]] : -1, s1 = (rm[1] & bit) && any_c ? pp[I[1]] : -1;
+                int s2 = (cm[0] & bit) && any_r ? pp[J[0]] + ss : -1, s3 = (cm[1] & bit) && any_r ? pp[J[1]] + ss : -1;
+                const int any = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : (s2 >= 0 ? s2 : s3));
+                slots[i] = make_int4(s0 >= 0 ? s0 : any, s1 >= 0 ? s1 : any, s2 >= 0 ? s2 : any, s3 >= 0 ? s3 : any);
+            }
+        }
+    }
+    float *C = mine ? tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)I[ia] * nt + J[jb]]) : nullptr;
+    floatx16 cre[4], cim[4];
+    if (mine) {
+        load_acc_tile(C, cre, cim, lane);
+    } else {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) cre[b] = cim[b] = floatx16{};
+    }
+    __syncthreads();
+    if (n) {
+        const int ua = ia, ub = 2 + jb;
+        qq_issue(lds0, sp, slots[0], 0, wave, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        // K quarters 0 and 2 of listed step i come from lds0, 1 and 3 from
+        // lds1; while one buffer is multiplied, the next quarter lands in the
+        // other (the last step's first quarter again after the end, read by no
+        // wave)
+        for (int i = 0; i < n; ++i) {
+            const uint32_t keep = (work >> i) & 1u ? 0xffffffffu : 0u;
+            const int4 cur = slots[i], nxt = slots[i + 1 < n ? i + 1 : i];
+#pragma unroll
+            for (int q = 0; q < 4; q += 2) {
+                qq_issue(lds1, sp, cur, q + 1, wave, lane);
+                __builtin_amdgcn_sched_barrier(0);  // the loads go out first
+                qq_mma(lds0, ua, ub, cre, cim, lane, THREE, keep);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                qq_issue(lds0, sp, q == 0 ? cur : nxt, q == 0 ? 2 : 0, wave, lane);
+                __builtin_amdgcn_sched_barrier(0);
+                qq_mma(lds1, ua, ub, cre, cim, lane, THREE, keep);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+        }
+    }
+    if (mine) store_acc_tile(C, cre, cim, lane);
+}
+
+template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                               const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                               const int32_t *);
+template __global__ void k_schur_quad_pp<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
+                                                const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                const int32_t *);
 
 }  // namespace aosx

@@ -80,14 +80,14 @@ __device__ __forceinline__ void store_acc_split_t(uint16_t *__restrict__ dst, co
             split_pair(cre[4 * q + e], rh[e], rl[e]);
             split_pair(cim[4 * q + e], ih[e], il[e]);
         }
-        const int off = c * TILE + row;
+        const int off = split_index(0, c, row);  // 4 consecutive k within one half
         auto pack = [](const uint16_t *v) {
             return make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
         };
         *reinterpret_cast<uint2 *>(dst + off) = pack(rh);
-        *reinterpret_cast<uint2 *>(dst + TILE2 + off) = pack(rl);
-        *reinterpret_cast<uint2 *>(dst + 2 * TILE2 + off) = pack(ih);
-        *reinterpret_cast<uint2 *>(dst + 3 * TILE2 + off) = pack(il);
+        *reinterpret_cast<uint2 *>(dst + SPLIT_HPLANE + off) = pack(rl);
+        *reinterpret_cast<uint2 *>(dst + 2 * SPLIT_HPLANE + off) = pack(ih);
+        *reinterpret_cast<uint2 *>(dst + 3 * SPLIT_HPLANE + off) = pack(il);
     }
 }
 

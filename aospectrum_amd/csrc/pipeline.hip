@@ -288,6 +288,7 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     if (const char *env = std::getenv("AOSX_SCHUR_GROUP")) group = std::min(32, std::max(1, std::atoi(env)));
     plan_schur(group);
     if (const char *env = std::getenv("AOSX_GRAPHS")) use_graphs = std::atoi(env) != 0;
+    if (const char *env = std::getenv("AOSX_QUAD_PIPELINE")) quad_pipelined = std::atoi(env) != 0;
     peak_bytes = resident_bytes;
 }
 
@@ -597,14 +598,11 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
                 const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
                 const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
                 const int32_t *qd = d_quads + 4 * grp_quad_ptr[g];
-                if (precision == AOSX_PRECISION_BF16X3_IR)
-                    k_schur_quad_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq,
-                                                                         nmat, d_tile_id, split, split_tiles, d_ppos,
-                                                                         d_step_s);
-                else
-                    k_schur_quad_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq,
-                                                                          nmat, d_tile_id, split, split_tiles,
-                                                                          d_ppos, d_step_s);
+                const bool three = precision == AOSX_PRECISION_BF16X3_IR;
+                auto quad = quad_pipelined ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
+                                           : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
+                quad<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat, d_tile_id, split,
+                                                 split_tiles, d_ppos, d_step_s);
                 AOSX_LAUNCHED();
             } else {
                 multi_update(g0, g1, grp_pair_ptr[g], np);

@@ -217,6 +217,48 @@ def test_graph_replays_match_eager_launches(small2d):
         eager.close()
 
 
+def _bundle_with_env(bundle, **env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return NativeBundle(bundle, device=0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision", ["bf16x3_ir", "bf16_ir"])
+def test_pipelined_quad_updates_match_step_kernel(small2d, precision):
+    """The trailing quad update pipelined by K halves accumulates the same
+    products in the same order as the step-at-a-time kernel: with step groups
+    of 4 (three trailing launches per factorisation) both give the same
+    energies and residuals."""
+    kpts = np.array([[0.13, 0.37, 0.0], [0.25, 0.1, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    settings = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision=precision,
+                            refinement_steps=1, refinement_passes=2)
+    step = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_QUAD_PIPELINE=0)
+    piped = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_QUAD_PIPELINE=1)
+    try:
+        piped.kernel_stats(True)
+        got, _ = band_solve(piped, settings, interval, kpoints=kpts)
+        ks = piped.kernel_stats(False)
+        ref, _ = band_solve(step, settings, interval, kpoints=kpts)
+        say("pipelined quad", precision, ks["schur_launches"], [r.energies_ev.size for r in got])
+        assert ks["schur_launches"] > 0
+        for a, b in zip(got, ref):
+            assert a.energies_ev.size == b.energies_ev.size
+            assert np.array_equal(a.energies_ev, b.energies_ev)
+            # the FP64 residual reduction order varies from run to run
+            assert a.maximum_residual_ev == pytest.approx(b.maximum_residual_ev, rel=1e-9)
+    finally:
+        piped.close()
+        step.close()
+
+
 def test_autotune_picks_a_timed_group_and_keeps_results(small2d, small2d_pattern):
     """The autotuner times every candidate step-group size and keeps the
     fastest; solves under each plan match the FP64 oracle."""
