@@ -118,6 +118,10 @@ struct Bundle {
     // the same trailing targets as 2 x 2 blocks (i0, i1, j0, j1; -1 absent)
     std::vector<int64_t> grp_quad_ptr;
     int32_t *d_quads = nullptr;
+    // Step masks, bit t - g0 for step t of the group: per pair (trailing and
+    // Crout lists alike) the steps with both operands L(i,t), U(t,j); per quad
+    // the steps with L(i0,t), L(i1,t), U(t,j0), U(t,j1) (four masks).
+    int32_t *d_pair_mask = nullptr, *d_quad_mask = nullptr;
     // BF16 precisions: the panels of a group also write split copies of their
     // tiles; L(i,t) sits in slot ppos[t*nt+i] of the group's split area and
     // U(t,j) in slot ppos[t*nt+j] + step_s[t] (split_tiles slots per matrix).

@@ -400,7 +400,8 @@ __global__ void __launch_bounds__(256) k_panel(float *__restrict__ tiles, int64_
 // contiguous run of the (matrix, pair) items, whose pairs share L rows and U
 // columns.  FP32 MFMA (the BF16 precisions use k_schur_group_split).
 __global__ void __launch_bounds__(256) k_schur_group(float *__restrict__ tiles, int64_t mat_stride, int nt, int g0,
-                                                    int g1, const int32_t *__restrict__ pairs, int npairs, int nmat,
+                                                    const int32_t *__restrict__ pairs,
+                                                    const int32_t *__restrict__ pair_mask, int npairs, int nmat,
                                                     const int32_t *__restrict__ tile_id) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
@@ -413,9 +414,9 @@ __global__ void __launch_bounds__(256) k_schur_group(float *__restrict__ tiles, 
     float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
     floatx16 cre, cim;
     load_acc(C, cre, cim, wave, lane);
-    for (int t = g0; t < g1; ++t) {
+    for (uint32_t mask = (uint32_t)pair_mask[q]; mask; mask &= mask - 1) {
+        const int t = g0 + __builtin_ctz(mask);
         const int32_t lid = tile_id[(int64_t)i * nt + t], uid = tile_id[(int64_t)t * nt + j];
-        if (lid < 0 || uid < 0) continue;
         __syncthreads();  // the previous step's products are done with LDS
         stage_a(tile_ptr(tiles, mat_stride, m, lid), Are, Aim, tid);
         stage_b(tile_ptr(tiles, mat_stride, m, uid), Bre, Bim, tid);
@@ -483,15 +484,28 @@ __device__ __forceinline__ void mma_split(const uint16_t *As, const uint16_t *Bs
 // with f = negation when `negate`, complex conjugation when `conj`.  Thread
 // tid converts the 4 x 4 block (rows 4*(tid/16).., columns 4*(tid%16)..) and
 // writes 4 runs of 4 consecutive K values per plane.
-__device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_t *S, bool transpose, bool negate,
-                                            bool conj, int tid) {
+// Thread tid's 4 x 4 block of an FP32 planar tile, as loaded from memory.
+struct TileBlock {
+    float4 re[4], im[4];
+};
+__device__ __forceinline__ TileBlock tile_block_load(const float *__restrict__ g, int tid) {
     const int br = tid >> 4, bc = tid & 15;
     const float4 *g4 = reinterpret_cast<const float4 *>(g);
+    TileBlock b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        b.re[q] = g4[(4 * br + q) * (TILE / 4) + bc];
+        b.im[q] = g4[PLANE / 4 + (4 * br + q) * (TILE / 4) + bc];
+    }
+    return b;
+}
+__device__ __forceinline__ void stage_split_block(const TileBlock &blk, uint16_t *S, bool transpose, bool negate,
+                                                  bool conj, int tid) {
+    const int br = tid >> 4, bc = tid & 15;
     float re[4][4], im[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float4 vr = g4[(4 * br + q) * (TILE / 4) + bc];
-        const float4 vi = g4[PLANE / 4 + (4 * br + q) * (TILE / 4) + bc];
+        const float4 vr = blk.re[q], vi = blk.im[q];
         re[q][0] = vr.x; re[q][1] = vr.y; re[q][2] = vr.z; re[q][3] = vr.w;
         im[q][0] = vi.x; im[q][1] = vi.y; im[q][2] = vi.z; im[q][3] = vi.w;
     }
@@ -518,6 +532,10 @@ __device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_
         *reinterpret_cast<uint2 *>(S + 3 * SPL + off) =
             make_uint2(il[0] | ((uint32_t)il[1] << 16), il[2] | ((uint32_t)il[3] << 16));
     }
+}
+__device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_t *S, bool transpose, bool negate,
+                                            bool conj, int tid) {
+    stage_split_block(tile_block_load(g, tid), S, transpose, negate, conj, tid);
 }
 
 // Panel step k in the BF16 precisions: the products of k_panel with both
@@ -593,9 +611,10 @@ __device__ __forceinline__ void put_split(uint16_t *dst, const u32x32 r, int tid
 
 template <bool THREE>
 __global__ void __launch_bounds__(256, 2) k_schur_group_split(
-    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ pairs,
-    int npairs, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
-    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ pairs,
+    const int32_t *__restrict__ pair_mask, int npairs, int nmat, const int32_t *__restrict__ tile_id,
+    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
+    const int32_t *__restrict__ step_s) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t total = (int64_t)npairs * nmat, per = (total + 7) / 8;
@@ -606,9 +625,7 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
     // steps of the group with both L(i,t) and U(t,j) present (groups <= 32 steps)
-    uint32_t mask = 0;
-    for (int t = g0; t < g1; ++t)
-        if (ppos[(int64_t)t * nt + i] >= 0 && ppos[(int64_t)t * nt + j] >= 0) mask |= 1u << (t - g0);
+    uint32_t mask = (uint32_t)pair_mask[q];
     floatx16 cre, cim;
     load_acc(C, cre, cim, wave, lane);
     if (mask) {
@@ -642,12 +659,12 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     store_acc(C, cre, cim, wave, lane);
 }
 
-template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                                   const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                                   const int32_t *);
-template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                                    const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                                    const int32_t *);
+template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, const int32_t *, const int32_t *,
+                                                   int, int, const int32_t *, const uint16_t *, int64_t,
+                                                   const int32_t *, const int32_t *);
+template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *,
+                                                    int, int, const int32_t *, const uint16_t *, int64_t,
+                                                    const int32_t *, const int32_t *);
 
 // Trailing update of a step group on 2 x 2 blocks of targets (BF16 precisions):
 // quad q = (i0, i1, j0, j1) (an index -1 is absent); wave w updates the full
@@ -730,9 +747,10 @@ __device__ __forceinline__ void store_acc_tile(float *__restrict__ t, const floa
 
 template <bool THREE>
 __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
-    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ quads,
-    int nquads, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
-    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
+    const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
+    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
+    const int32_t *__restrict__ step_s) {
     __shared__ __attribute__((aligned(16))) uint16_t A0[4 * SPL], A1[4 * SPL], B0[4 * SPL], B1[4 * SPL];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
@@ -742,15 +760,8 @@ __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
     const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
     // step masks: rows (L present), columns (U present), this wave's target
-    uint32_t rm[2] = {0u, 0u}, cm[2] = {0u, 0u};
-    for (int t = g0; t < g1; ++t) {
-        const int32_t *pp = ppos + (int64_t)t * nt;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (I[h] >= 0 && pp[I[h]] >= 0) rm[h] |= 1u << (t - g0);
-            if (J[h] >= 0 && pp[J[h]] >= 0) cm[h] |= 1u << (t - g0);
-        }
-    }
+    const int4 qm = reinterpret_cast<const int4 *>(quad_mask)[q];
+    const uint32_t rm[2] = {(uint32_t)qm.x, (uint32_t)qm.y}, cm[2] = {(uint32_t)qm.z, (uint32_t)qm.w};
     const int ia = wave >> 1, jb = wave & 1;
     uint32_t mine = rm[ia] & cm[jb];
     // steps where some wave has work, and the tiles they need
@@ -826,12 +837,10 @@ __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
     if (mine) store_acc_tile(C, cre, cim, lane);
 }
 
-template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                                  const int32_t *);
-template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                                   const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                                   const int32_t *);
+template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                                 const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
 
 
 // Trailing update of a step group on 2 x 2 blocks of targets, pipelined by K
@@ -910,9 +919,10 @@ __device__ __forceinline__ void qq_mma(const uint16_t *buf, int ua, int ub, floa
 
 template <bool THREE>
 __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
-    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, int g1, const int32_t *__restrict__ quads,
-    int nquads, int nmat, const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
-    int64_t split_stride, const int32_t *__restrict__ ppos, const int32_t *__restrict__ step_s) {
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
+    const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
+    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
+    const int32_t *__restrict__ step_s) {
     // two buffers as two LDS objects: the compiler then knows that the reads
     // of one do not wait for the direct loads into the other
     __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
@@ -925,15 +935,8 @@ __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
     const int m = (int)(item / nquads), q = (int)(item % nquads);
     const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
-    uint32_t rm[2] = {0u, 0u}, cm[2] = {0u, 0u};
-    for (int t = g0; t < g1; ++t) {
-        const int32_t *pp = ppos + (int64_t)t * nt;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (I[h] >= 0 && pp[I[h]] >= 0) rm[h] |= 1u << (t - g0);
-            if (J[h] >= 0 && pp[J[h]] >= 0) cm[h] |= 1u << (t - g0);
-        }
-    }
+    const int4 qm = reinterpret_cast<const int4 *>(quad_mask)[q];
+    const uint32_t rm[2] = {(uint32_t)qm.x, (uint32_t)qm.y}, cm[2] = {(uint32_t)qm.z, (uint32_t)qm.w};
     const int ia = wave >> 1, jb = wave & 1;
     const uint32_t mine = rm[ia] & cm[jb];
     const uint32_t all = (rm[0] & cm[0]) | (rm[0] & cm[1]) | (rm[1] & cm[0]) | (rm[1] & cm[1]);
@@ -1000,11 +1003,9 @@ __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
     if (mine) store_acc_tile(C, cre, cim, lane);
 }
 
-template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                               const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                               const int32_t *);
-template __global__ void k_schur_quad_pp<false>(float *, int64_t, int, int, int, const int32_t *, int, int,
-                                                const int32_t *, const uint16_t *, int64_t, const int32_t *,
-                                                const int32_t *);
+template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                              const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+template __global__ void k_schur_quad_pp<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                               const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
 
 }  // namespace aosx

@@ -197,23 +197,60 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
 #pragma unroll
     for (int c = 0; c < NC; ++c)
         if (c < nc) load_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
-    for (int k = g0; k < g1; ++k) {
+    if (SPLIT) {
+        // the next present tile and its first RHS block are loaded into
+        // registers while the current products run
+        auto tile_of = [&](int k) {
+            return ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
+        };
+        auto next_k = [&](int k) {
+            while (k < g1 && tile_of(k) < 0) ++k;
+            return k;
+        };
+        const float *tm = tiles + (int64_t)m * mat_stride;
+        int k = next_k(g0);
+        TileBlock tn;
+        u32x32 yn;
+        if (k < g1) {
+            tn = tile_block_load(tm + (int64_t)tile_of(k) * TILE_FLOATS, tid);
+            yn = fetch_split(yslot(ysplit, m, k, nchunk, c0), tid);
+        }
+        while (k < g1) {
+            const int kn = next_k(k + 1);
+            __syncthreads();  // the previous products are done with LDS
+            stage_split_block(tn, As, ctrans != 0, true, ctrans != 0, tid);
+            put_split(Bs, yn, tid);
+            if (kn < g1) {
+                tn = tile_block_load(tm + (int64_t)tile_of(kn) * TILE_FLOATS, tid);
+                yn = fetch_split(yslot(ysplit, m, kn, nchunk, c0), tid);
+            }
+            __syncthreads();
+            mma_split<true>(As, Bs, cre[0], cim[0], wave, lane);
+#pragma unroll
+            for (int c = 1; c < NC; ++c) {
+                if (c >= nc) break;
+                __syncthreads();
+                put_split(Bs, fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid), tid);
+                __syncthreads();
+                mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
+            }
+            k = kn;
+        }
+    }
+    for (int k = g0; !SPLIT && k < g1; ++k) {
         const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
         if (id < 0) continue;
         const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
         __syncthreads();  // the previous products are done with LDS
-        if (SPLIT) stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
-        else if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
+        if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
         else stage_a(t, Are, Aim, tid);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             if (c >= nc) break;
             if (c > 0) __syncthreads();
-            if (SPLIT) put_split(Bs, fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid), tid);
-            else stage_b(y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c), Bre, Bim, tid);
+            stage_b(y + rhs_block(y_mat_stride, m, k, nchunk, c0 + c), Bre, Bim, tid);
             __syncthreads();
-            if (SPLIT) mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
-            else mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
+            mma_fp32<true>(Are, Aim, Bre, Bim, cre[c], cim[c], wave, lane);
         }
     }
 #pragma unroll

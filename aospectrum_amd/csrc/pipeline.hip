@@ -184,7 +184,7 @@ Bundle::~Bundle() {
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
                         (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
                         (void *)d_atom_off, (void *)d_pairs, (void *)d_quads, (void *)d_ppos, (void *)d_step_s,
-                        (void *)d_fwd_rows, (void *)d_bwd_rows})
+                        (void *)d_fwd_rows, (void *)d_bwd_rows, (void *)d_pair_mask, (void *)d_quad_mask})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -303,7 +303,7 @@ void Bundle::plan_schur(int group) {
     grp_pair_ptr.assign(1, 0);
     grp_updates.clear();
     grp_tiles.clear();
-    std::vector<int32_t> pairs, quads;
+    std::vector<int32_t> pairs, quads, pair_mask, quad_mask;
     grp_quad_ptr.assign(1, 0);
     std::vector<uint8_t> mark((size_t)nt * nt, 0);
     std::vector<int32_t> ppos((size_t)nt * nt, -1), step_s(nt, 0);
@@ -333,6 +333,13 @@ void Bundle::plan_schur(int group) {
         rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
         const int r = (int)rows.size();
         auto marked = [&](int a, int b) { return a < r && b < r && mark[(size_t)rows[a] * nt + rows[b]] != 0; };
+        // steps of the group where tile index x has an L (equivalently U) tile
+        auto steps_of = [&](int x) {
+            uint32_t mk = 0;
+            for (int t = g0; x >= 0 && t < g1; ++t)
+                if (ppos[(size_t)t * nt + x] >= 0) mk |= 1u << (t - g0);
+            return mk;
+        };
         for (int bi = 0; bi < r; bi += 8)
             for (int bj = 0; bj < r; bj += 8)
                 for (int a = bi; a < std::min(r, bi + 8); ++a)
@@ -340,6 +347,7 @@ void Bundle::plan_schur(int group) {
                         if (marked(a, b)) {
                             pairs.push_back(rows[a]);
                             pairs.push_back(rows[b]);
+                            pair_mask.push_back((int32_t)(steps_of(rows[a]) & steps_of(rows[b])));
                         }
         // quads: 2 x 2 blocks of consecutive rows and columns, in 4 x 4 blocks
         // of quads; -1 pads an odd row count
@@ -348,10 +356,12 @@ void Bundle::plan_schur(int group) {
                 for (int a = bi; a < std::min(r, bi + 8); a += 2)
                     for (int b = bj; b < std::min(r, bj + 8); b += 2)
                         if (marked(a, b) || marked(a, b + 1) || marked(a + 1, b) || marked(a + 1, b + 1)) {
-                            quads.push_back(rows[a]);
-                            quads.push_back(a + 1 < r ? rows[a + 1] : -1);
-                            quads.push_back(rows[b]);
-                            quads.push_back(b + 1 < r ? rows[b + 1] : -1);
+                            const int q4[4] = {rows[a], a + 1 < r ? rows[a + 1] : -1, rows[b],
+                                               b + 1 < r ? rows[b + 1] : -1};
+                            for (int e = 0; e < 4; ++e) {
+                                quads.push_back(q4[e]);
+                                quad_mask.push_back((int32_t)steps_of(q4[e]));
+                            }
                         }
         for (int a = 0; a < r; ++a)
             for (int b = 0; b < r; ++b) mark[(size_t)rows[a] * nt + rows[b]] = 0;
@@ -377,11 +387,16 @@ void Bundle::plan_schur(int group) {
             };
             auto add = [&](int i, int j) {
                 int cnt = 0;
+                uint32_t mk = 0;
                 for (int tp = g0; tp < t; ++tp)
-                    if (has(tp, i) && has(tp, j)) ++cnt;
+                    if (has(tp, i) && has(tp, j)) {
+                        ++cnt;
+                        mk |= 1u << (tp - g0);
+                    }
                 if (cnt == 0) return;
                 pairs.push_back(i);
                 pairs.push_back(j);
+                pair_mask.push_back((int32_t)mk);
                 crout_updates[t] += cnt;
             };
             add(t, t);
@@ -413,13 +428,15 @@ void Bundle::plan_schur(int group) {
     }
     // the launch sequences captured so far follow the old plan
     clear_graphs();
-    for (int32_t *p : {d_pairs, d_quads, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows})
+    for (int32_t *p : {d_pairs, d_quads, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows, d_pair_mask, d_quad_mask})
         if (p) AOSX_HIP(hipFree(p));
     uint64_t pb = 0;
     d_quads = upload_vec(quads, pb);
     d_fwd_rows = upload_vec(frows, pb);
     d_bwd_rows = upload_vec(brows, pb);
     d_pairs = upload_vec(pairs, pb);
+    d_pair_mask = upload_vec(pair_mask, pb);
+    d_quad_mask = upload_vec(quad_mask, pb);
     d_ppos = upload_vec(ppos, pb);
     d_step_s = upload_vec(step_s, pb);
     resident_bytes = resident_bytes - plan_bytes + pb;
@@ -542,15 +559,15 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
     auto multi_update = [&](int t0, int t1, int64_t first, int64_t np) {
         const int64_t items = np * nmat;
         const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
-        const int32_t *pr = d_pairs + 2 * first;
+        const int32_t *pr = d_pairs + 2 * first, *pm = d_pair_mask + first;
         if (precision == AOSX_PRECISION_BF16X3_IR)
-            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat,
+            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat,
                                                                   d_tile_id, split, split_tiles, d_ppos, d_step_s);
         else if (precision == AOSX_PRECISION_BF16_IR)
-            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat,
+            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat,
                                                                    d_tile_id, split, split_tiles, d_ppos, d_step_s);
         else
-            k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, t1, pr, (int)np, nmat, d_tile_id);
+            k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat, d_tile_id);
         AOSX_LAUNCHED();
     };
     auto launches = [&]() {
@@ -597,11 +614,11 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             if (split_mode) {
                 const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
                 const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
-                const int32_t *qd = d_quads + 4 * grp_quad_ptr[g];
+                const int32_t *qd = d_quads + 4 * grp_quad_ptr[g], *qm = d_quad_mask + 4 * grp_quad_ptr[g];
                 const bool three = precision == AOSX_PRECISION_BF16X3_IR;
                 auto quad = quad_pipelined ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
                                            : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
-                quad<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, g1, qd, (int)nq, nmat, d_tile_id, split,
+                quad<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, qd, qm, (int)nq, nmat, d_tile_id, split,
                                                  split_tiles, d_ppos, d_step_s);
                 AOSX_LAUNCHED();
             } else {
