@@ -192,6 +192,9 @@ class BandRequest(C.Structure):
         ("maximum_residual_ev", _F64P),
         ("subspace_rank", _I32P),
         ("saturated", _I32P),
+        ("eigenvectors", _C64P),
+        ("pass_rank", _I32P),
+        ("pass_width", _I32P),
     ]
 
 
@@ -262,6 +265,22 @@ class AutotuneResult(C.Structure):
     ]
 
 
+class ResidualRequest(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32),
+        ("struct_size", C.c_uint32),
+        ("n_kpoints", C.c_int64),
+        ("k_fractional", _F64P),
+        ("h_data", _C64P),
+        ("s_data", _C64P),
+        ("count", C.c_int32),
+        ("reserved", C.c_int32),
+        ("energies_ev", _F64P),
+        ("vectors", _F64P),
+        ("residual_ev", _F64P),
+    ]
+
+
 _STRUCTS = (
     BundleRequest,
     BundleResult,
@@ -274,6 +293,7 @@ _STRUCTS = (
     KernelStats,
     AutotuneRequest,
     AutotuneResult,
+    ResidualRequest,
 )
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "csrc" / "libaosx_hip.so"
@@ -316,6 +336,7 @@ def load() -> C.CDLL:
         "aosx_kernel_stats_read_v1": [_P, C.c_int32, C.POINTER(KernelStats)],
         "aosx_bundle_trim_v1": [_P, C.POINTER(C.c_uint64)],
         "aosx_bundle_autotune_v1": [_P, C.POINTER(AutotuneRequest), C.POINTER(AutotuneResult)],
+        "aosx_eig_residual_v1": [_P, C.POINTER(ResidualRequest), C.POINTER(StatusResult)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -336,6 +357,7 @@ EXPORTED_SYMBOLS = (
     "aosx_struct_sizes_v1",
     "aosx_bundle_trim_v1",
     "aosx_bundle_autotune_v1",
+    "aosx_eig_residual_v1",
 )
 
 

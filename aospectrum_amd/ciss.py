@@ -46,6 +46,11 @@ class CissResult:
     warnings: tuple[str, ...]
     complete: bool
     subspace_rank: int = 0
+    pass_rank: int = 0
+    pass_width: int = 0
+    # the selected Ritz vectors [n, m] (orbital order, S-normalised) when
+    # requested with return_vectors=True
+    eigenvectors: np.ndarray | None = None
 
 
 @dataclass(slots=True)
@@ -177,6 +182,7 @@ def band_solve(
     h_data: np.ndarray | None = None,
     s_data: np.ndarray | None = None,
     intervals: np.ndarray | None = None,
+    return_vectors: bool = False,
 ) -> tuple[list[CissResult], dict[str, Any]]:
     """Run the CISS Band path for a batch of k points (on-device assembly) or
     of explicit H/S value arrays on the bundle's pattern.  ``intervals``
@@ -186,13 +192,14 @@ def band_solve(
     With ``refinement_policy = "on_demand"`` the batch is first solved without
     iterative refinement; the k points whose FP64 eigenpair residual exceeds
     ``residual_target_ev`` are solved again with ``refinement_steps`` steps and
-    their results replace the first ones (``info["refined"]`` lists them)."""
+    their results replace the first ones (``info["refined"]`` lists them).
+    ``return_vectors`` also returns each k point's selected Ritz vectors."""
     if settings.refinement_policy == "fixed" or settings.refinement_steps == 0:
-        results, info = _band_call(native, settings, interval, kpoints, h_data, s_data, intervals)
+        results, info = _band_call(native, settings, interval, kpoints, h_data, s_data, intervals, return_vectors)
         info["refined"] = []
         return results, info
     first = replace(settings, refinement_steps=0, refinement_policy="fixed")
-    results, info = _band_call(native, first, interval, kpoints, h_data, s_data, intervals)
+    results, info = _band_call(native, first, interval, kpoints, h_data, s_data, intervals, return_vectors)
     redo = [i for i, r in enumerate(results) if r.maximum_residual_ev > settings.residual_target_ev]
     info["refined"] = redo
     if redo:
@@ -206,6 +213,7 @@ def band_solve(
             None if h_data is None else np.asarray(h_data).reshape(len(results), -1)[sel],
             None if s_data is None else np.asarray(s_data).reshape(len(results), -1)[sel],
             None if intervals is None else np.asarray(intervals, dtype=np.float64).reshape(-1, 2)[sel],
+            return_vectors,
         )
         for i, r in zip(redo, again):
             results[i] = r
@@ -218,7 +226,7 @@ def band_solve(
     return results, info
 
 
-def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
+def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals, return_vectors):
     lib = N.load()
     if (kpoints is None) == (h_data is None):
         raise InputError("give either k points or explicit H/S values")
@@ -258,6 +266,9 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
     resid = np.zeros(nk, dtype=np.float64)
     rank = np.zeros(nk, dtype=np.int32)
     sat = np.zeros(nk, dtype=np.int32)
+    prank = np.zeros(nk, dtype=np.int32)
+    pwidth = np.zeros(nk, dtype=np.int32)
+    vecs = np.zeros((nk, native.n, cap), dtype=np.complex64) if return_vectors else None
     req.n_kpoints = nk
     req.lower_ev = interval.lower_ev
     req.upper_ev = interval.upper_ev
@@ -282,6 +293,10 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
     req.maximum_residual_ev = N.ptr(resid, C.c_double)
     req.subspace_rank = N.ptr(rank, C.c_int32)
     req.saturated = N.ptr(sat, C.c_int32)
+    req.pass_rank = N.ptr(prank, C.c_int32)
+    req.pass_width = N.ptr(pwidth, C.c_int32)
+    if vecs is not None:
+        req.eigenvectors = N.ptr(vecs, N.Complex64)
     res = N.BandResult()
     rc = lib.aosx_band_solve_v1(native.handle, C.byref(req), C.byref(res))
     N.check(rc, res, "CISS band solve")
@@ -299,6 +314,9 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
                 warnings=_warnings(float(resid[i]), bool(sat[i])),
                 complete=not bool(sat[i]),
                 subspace_rank=int(rank[i]),
+                pass_rank=int(prank[i]),
+                pass_width=int(pwidth[i]),
+                eigenvectors=None if vecs is None else vecs[i, :, : counts[i]].copy(),
             )
         )
     info = {
@@ -309,6 +327,51 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals):
         "wall_seconds": wall,
     }
     return results, info
+
+
+def eig_residual(
+    native: NativeBundle,
+    energies_ev: np.ndarray,
+    vectors: np.ndarray,
+    *,
+    kpoints: np.ndarray | None = None,
+    h_data: np.ndarray | None = None,
+    s_data: np.ndarray | None = None,
+) -> np.ndarray:
+    """||H(k) v_j - e_j S(k) v_j||_2 in FP64 on the device, the residual of
+    ciss.py:362-372 for caller-given pairs: ``energies_ev`` [nk, m],
+    ``vectors`` [nk, n, m] (converted to complex128)."""
+    lib = N.load()
+    if (kpoints is None) == (h_data is None):
+        raise InputError("give either k points or explicit H/S values")
+    e = np.ascontiguousarray(np.asarray(energies_ev, dtype=np.float64))
+    if e.ndim == 1:
+        e = e[None, :]
+    nk, m = e.shape
+    v = np.ascontiguousarray(np.asarray(vectors, dtype=np.complex128).reshape(nk, native.n, m))
+    out = np.zeros((nk, m), dtype=np.float64)
+    req = N.new(N.ResidualRequest)
+    keep = [e, v]
+    if kpoints is not None:
+        k = np.ascontiguousarray(np.asarray(kpoints, dtype=np.float64).reshape(nk, 3))
+        keep.append(k)
+        req.k_fractional = N.ptr(k, C.c_double)
+    else:
+        h = np.ascontiguousarray(np.asarray(h_data, dtype=np.complex64).reshape(nk, native.nnz))
+        s = np.ascontiguousarray(np.asarray(s_data, dtype=np.complex64).reshape(nk, native.nnz))
+        keep += [h, s]
+        req.h_data = N.ptr(h, N.Complex64)
+        req.s_data = N.ptr(s, N.Complex64)
+    req.n_kpoints = nk
+    req.count = m
+    req.energies_ev = N.ptr(e, C.c_double)
+    req.vectors = v.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    req.residual_ev = N.ptr(out, C.c_double)
+    res = N.StatusResult()
+    rc = lib.aosx_eig_residual_v1(native.handle, C.byref(req), C.byref(res))
+    del keep
+    N.check(rc, res, "eigenpair residual")
+    return out
 
 
 def _require_shared_pattern(matrices: SparsePair) -> None:

@@ -268,8 +268,8 @@ int aosx_shift_solve_v1(aosx_bundle *bundle, const aosx_shift_request_v1 *q, aos
         for (int i = 0; i < nb; ++i) z[i] = make_float2(q->shifts[i].real, q->shifts[i].imag);
         float *tiles = b.buf<float>("tiles", (size_t)nb * b.factor_floats());
         float *dinv = b.buf<float>("dinv", (size_t)nb * b.ts.n_diag * 4 * 64 * 64);
-        int *d_repl = b.buf<int>("replaced", 65536);
-        b.zero(d_repl, 65536 * sizeof(int));
+        int *d_repl = b.buf<int>("replaced", (size_t)std::max(nb, 256));
+        b.zero(d_repl, (size_t)(nb + 3) / 4 * 16);
         b.factor(nb, kidx, z, q->precision, tiles, dinv, d_repl);
         AOSX_HIP(hipStreamSynchronize(b.stream));
         double t2 = now();
@@ -364,6 +364,7 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         p.conjugate_pairs = q->conjugate_pairs;
         p.resident_limit = q->resident_limit;
         p.intervals = q->intervals;
+        p.want_vectors = q->eigenvectors != nullptr;
         aosx::CissOutput o;
         b.band(p, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),
                reinterpret_cast<const float2 *>(q->s_data), reinterpret_cast<const float2 *>(q->probes),
@@ -376,11 +377,49 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
             q->maximum_residual_ev[k] = o.max_residual[k];
             q->subspace_rank[k] = o.rank[k];
             q->saturated[k] = o.saturated[k];
+            if (q->pass_rank) q->pass_rank[k] = o.pass_rank[k];
+            if (q->pass_width) q->pass_width[k] = o.pass_width[k];
+            if (q->eigenvectors) {
+                const int cap = q->energy_capacity, c = (int)std::min<size_t>(e.size(), (size_t)cap);
+                aosx_complex64 *dst = q->eigenvectors + (size_t)k * b.n * cap;
+                const std::vector<float2> &v = o.vectors[k];
+                for (int64_t r = 0; r < b.n; ++r)
+                    for (int j = 0; j < cap; ++j)
+                        dst[(size_t)r * cap + j] = j < c ? aosx_complex64{v[(size_t)r * e.size() + j].x,
+                                                                         v[(size_t)r * e.size() + j].y}
+                                                         : aosx_complex64{0.f, 0.f};
+            }
         }
         for (int i = 0; i < AOSX_STAGE_COUNT; ++i) res->stage_seconds[i] = o.stages[i];
         res->replaced_pivots = o.replaced_pivots;
         res->factorizations = o.factorizations;
         res->peak_device_bytes = b.peak_bytes;
+        return 0;
+    });
+}
+
+int aosx_eig_residual_v1(aosx_bundle *bundle, const aosx_residual_request_v1 *q, aosx_status_v1 *res) {
+    if (!q || !res) return 1;
+    begin(res);
+    if (!abi_ok(q)) return fail(res, AOSX_ABI_MISMATCH, 0, "request ABI mismatch");
+    if (!bundle || !q->energies_ev || !q->vectors || !q->residual_ev || (!q->k_fractional && (!q->h_data || !q->s_data)))
+        return fail(res, AOSX_NULL_POINTER, 0, "required input or output pointer is null");
+    Bundle &b = *bundle->impl;
+    if (b.device < 0) return fail(res, AOSX_NO_DEVICE, 0, "bundle was created without a device");
+    if (q->n_kpoints <= 0 || q->n_kpoints > 64 || q->count < 1 || q->count > 256)
+        return fail(res, AOSX_BATCH_DIMENSIONS, 0, "k-point count must be in [1, 64] and count in [1, 256]");
+    if (q->k_fractional) {
+        for (int64_t i = 0; i < 3 * q->n_kpoints; ++i)
+            if (!std::isfinite(q->k_fractional[i])) return fail(res, AOSX_NONFINITE_VALUES, 0, "k point is not finite");
+    } else if (!finite_c64(q->h_data, q->n_kpoints * b.nnz) || !finite_c64(q->s_data, q->n_kpoints * b.nnz)) {
+        return fail(res, AOSX_NONFINITE_VALUES, 0, "H/S values contain non-finite entries");
+    }
+    return guarded(res, [&]() -> int {
+        const double t0 = now();
+        b.eig_residual(q->n_kpoints, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),
+                       reinterpret_cast<const float2 *>(q->s_data), q->count, q->energies_ev, q->vectors,
+                       q->residual_ev);
+        res->seconds = now() - t0;
         return 0;
     });
 }
@@ -451,7 +490,7 @@ int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity) {
                            (int64_t)sizeof(aosx_shift_request_v1),  (int64_t)sizeof(aosx_shift_result_v1),
                            (int64_t)sizeof(aosx_band_request_v1),   (int64_t)sizeof(aosx_band_result_v1),
                            (int64_t)sizeof(aosx_kernel_stats_v1),   (int64_t)sizeof(aosx_autotune_request_v1),
-                           (int64_t)sizeof(aosx_autotune_result_v1)};
+                           (int64_t)sizeof(aosx_autotune_result_v1), (int64_t)sizeof(aosx_residual_request_v1)};
     const int count = (int)(sizeof(all) / sizeof(all[0]));
     for (int i = 0; i < count && i < capacity; ++i) sizes[i] = all[i];
     return count;

@@ -79,6 +79,7 @@ struct CissParams {
     int conjugate_pairs = 0;
     int resident_limit = 0;  // 0: keep factorisations resident whenever they fit
     const double *intervals = nullptr;  // [nk][2] per-k absolute intervals, or null
+    bool want_vectors = false;          // return the selected Ritz vectors
 };
 
 struct CissOutput {
@@ -86,6 +87,8 @@ struct CissOutput {
     std::vector<double> max_residual;
     std::vector<int> rank;
     std::vector<int> saturated;
+    std::vector<int> pass_rank, pass_width;      // last filter pass (0 without passes)
+    std::vector<std::vector<float2>> vectors;    // per k: [n][count] when requested
     double stages[AOSX_STAGE_COUNT] = {0};
     int64_t replaced_pivots = 0;
     int64_t factorizations = 0;
@@ -199,6 +202,10 @@ struct Bundle {
                     const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                     const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
                     float2 *x_out, double *rel_residual, int precision);
+    // norm[k][j] = ||H_k v_kj - e_kj S_k v_kj||_2 (FP64) for count pairs per k;
+    // H_k, S_k assembled at kpts or given explicitly (host arrays)
+    void eig_residual(int64_t nk, const double *kpts, const float2 *h_in, const float2 *s_in, int count,
+                      const double *energies, const double *vectors, double *norm);
     void band(const CissParams &p, const double *kpts, const float2 *h_in, const float2 *s_in,
               const float2 *probes, const float2 *augmentation, int augmentation_width, CissOutput &out);
 };

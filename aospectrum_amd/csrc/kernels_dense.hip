@@ -6,8 +6,6 @@
 //   moments     M[:, mL:(m+1)L] += coef[b][m] X_b           (ciss.py:324-333)
 //   gram64      G = A^H B in FP64 (two-stage, deterministic order)
 //   tall_gemm   Y = X T with T small (FP64), output complex64 or complex128
-//   jacobi      Hermitian eigendecomposition, FP64, cyclic two-sided Jacobi with
-//               round-robin ordering (r/2 disjoint rotations per round)
 //   cholesky / trinv / small_gemm for the projected generalized problem
 //   eig_residual ||H v - e S v||_2 per column in FP64
 #include "device_common.h"
@@ -308,139 +306,6 @@ template __global__ void k_tall_gemm<float2>(float2 *, int64_t, const float2 *, 
                                              int64_t, int, int64_t);
 template __global__ void k_tall_gemm<double2>(double2 *, int64_t, const float2 *, int64_t, int, const c64 *,
                                               int, int64_t, int, int64_t);
-
-// ---------------------------------------------------------------- Jacobi
-// One workgroup per problem.  A (r x r, leading dimension ld) is destroyed;
-// on exit w holds ascending eigenvalues and Vout the matching eigenvectors
-// (columns).  Rotation for the pivot pair (p, q) with b = A_pq = |b| e^{i phi}:
-// G = diag(1, e^{-i phi}) [[c, s], [-s, c]] with the real Jacobi angle of
-// [[a_pp, |b|], [|b|, a_qq]] (Golub & Van Loan, sym.schur2); A <- G^H A G.
-__global__ void __launch_bounds__(1024) k_jacobi(c64 *__restrict__ A_all, c64 *__restrict__ V_all,
-                                                c64 *__restrict__ Vout_all, double *__restrict__ w_all,
-                                                const int32_t *__restrict__ rk, int ld, int64_t stride,
-                                                int max_sweeps, double tol, int *__restrict__ sweeps_out) {
-    __shared__ double rc[128], rs[128], rpr[128], rpi[128];
-    __shared__ int rp[128], rq[128];
-    __shared__ double red[1024], red2[1024];
-    __shared__ int done;
-    const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
-    const int r = rk[k];
-    c64 *A = A_all + k * stride;
-    c64 *V = V_all + k * stride;
-    c64 *Vout = Vout_all + k * stride;
-    double *w = w_all + (int64_t)k * ld;
-    for (int e = tid; e < r * r; e += nthr) {
-        const int i = e / r, j = e % r;
-        V[(int64_t)i * ld + j] = {i == j ? 1.0 : 0.0, 0.0};
-    }
-    const int rr = r + (r & 1);
-    const int npairs = rr / 2;
-    int sweep = 0;
-    __syncthreads();
-    // element loops walk (row, column) or (row, pair) indices incrementally:
-    // thread tid starts at flat index tid and advances by nthr
-    auto start = [&](int width, int &a, int &b, int &da, int &db) {
-        a = tid / width;
-        b = tid % width;
-        da = nthr / width;
-        db = nthr % width;
-    };
-    for (; sweep < max_sweeps; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        {
-            int i, j, di, dj;
-            start(r, i, j, di, dj);
-            for (; i < r; i += di, j += dj) {
-                if (j >= r) { j -= r; ++i; if (i >= r) break; }
-                const double v = cabs2(A[(int64_t)i * ld + j]);
-                tot += v;
-                if (i != j) off += v;
-            }
-        }
-        red[tid] = off;
-        red2[tid] = tot;
-        __syncthreads();
-        for (int s2 = nthr / 2; s2 > 0; s2 >>= 1) {
-            if (tid < s2) { red[tid] += red[tid + s2]; red2[tid] += red2[tid + s2]; }
-            __syncthreads();
-        }
-        if (tid == 0) done = !(red[0] > tol * tol * red2[0]);
-        __syncthreads();
-        if (done) break;
-        for (int t = 0; t < rr - 1; ++t) {
-            if (tid < npairs) {
-                int p, q;
-                if (tid == 0) { p = rr - 1; q = t; }
-                else { p = (t + tid) % (rr - 1); q = (t - tid + (rr - 1)) % (rr - 1); }
-                if (p > q) { const int tmp = p; p = q; q = tmp; }
-                double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
-                if (q < r) {
-                    const c64 b = A[(int64_t)p * ld + q];
-                    const double mb = sqrt(cabs2(b));
-                    const double app = A[(int64_t)p * ld + p].re, aqq = A[(int64_t)q * ld + q].re;
-                    if (mb > 1e-300 && mb > 1e-18 * sqrt(fabs(app * aqq))) {
-                        const double tau = (aqq - app) / (2.0 * mb);
-                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + tt * tt);
-                        s = tt * c;
-                        er = b.re / mb;
-                        ei = b.im / mb;  // e^{i phi}
-                    }
-                } else {
-                    q = p;  // dummy partner: no rotation
-                }
-                rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = s; rpr[tid] = er; rpi[tid] = ei;
-            }
-            __syncthreads();
-            // columns: A <- A G, V <- V G
-            int i, pi, di, dpi;
-            start(npairs, i, pi, di, dpi);
-            for (; i < r; i += di, pi += dpi) {
-                if (pi >= npairs) { pi -= npairs; ++i; if (i >= r) break; }
-                const int p = rp[pi], q = rq[pi];
-                if (p == q || rs[pi] == 0.0) continue;
-                const double c = rc[pi], s = rs[pi];
-                const c64 em = {rpr[pi], -rpi[pi]};  // e^{-i phi}
-                c64 *ap = A + (int64_t)i * ld + p, *aq = A + (int64_t)i * ld + q;
-                const c64 x = *ap, y = cmul(em, *aq);
-                *ap = {c * x.re - s * y.re, c * x.im - s * y.im};
-                *aq = {s * x.re + c * y.re, s * x.im + c * y.im};
-                c64 *vp = V + (int64_t)i * ld + p, *vq = V + (int64_t)i * ld + q;
-                const c64 xv = *vp, yv = cmul(em, *vq);
-                *vp = {c * xv.re - s * yv.re, c * xv.im - s * yv.im};
-                *vq = {s * xv.re + c * yv.re, s * xv.im + c * yv.im};
-            }
-            __syncthreads();
-            // rows: A <- G^H A
-            int qi, j, dqi, dj;
-            start(r, qi, j, dqi, dj);
-            for (; qi < npairs; qi += dqi, j += dj) {
-                if (j >= r) { j -= r; ++qi; if (qi >= npairs) break; }
-                const int p = rp[qi], q = rq[qi];
-                if (p == q || rs[qi] == 0.0) continue;
-                const double c = rc[qi], s = rs[qi];
-                const c64 ep = {rpr[qi], rpi[qi]};  // e^{+i phi}
-                c64 *ap = A + (int64_t)p * ld + j, *aq = A + (int64_t)q * ld + j;
-                const c64 x = *ap, y = cmul(ep, *aq);
-                *ap = {c * x.re - s * y.re, c * x.im - s * y.im};
-                *aq = {s * x.re + c * y.re, s * x.im + c * y.im};
-            }
-            __syncthreads();
-        }
-    }
-    if (tid == 0 && sweeps_out) sweeps_out[k] = sweep;
-    // ascending order by rank counting (stable)
-    for (int i = tid; i < r; i += nthr) {
-        const double wi = A[(int64_t)i * ld + i].re;
-        int rank = 0;
-        for (int j = 0; j < r; ++j) {
-            const double wj = A[(int64_t)j * ld + j].re;
-            rank += (wj < wi) || (wj == wi && j < i);
-        }
-        w[rank] = wi;
-        for (int row = 0; row < r; ++row) Vout[(int64_t)row * ld + rank] = V[(int64_t)row * ld + i];
-    }
-}
 
 // Lower Cholesky in place (upper part zeroed); status[k] = 1 when not positive definite.
 __global__ void __launch_bounds__(1024) k_cholesky(c64 *__restrict__ A_all, const int32_t *__restrict__ rk,

@@ -1,0 +1,250 @@
+// Batched Hermitian eigendecomposition of the small FP64 problems of the CISS
+// subspace stage (the compressed moment Gram and the projected Rayleigh-Ritz
+// matrix, r <= 256): cyclic two-sided Jacobi with round-robin ordering.
+//
+// Round t of a sweep pairs the rr = r + (r odd) indices into rr/2 disjoint
+// pairs (the circle method: index rr-1 stays, the others rotate).  Every pair
+// (p, q), p < q, gets the rotation M = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
+// that zeroes A(p, q) of the 2 x 2 block, and A <- M^H A M, V <- V M for all
+// pairs at once.  The work is split over two kernels:
+//   k_jacobi_rotate   one workgroup per problem.  A is kept as its packed upper
+//                     triangle -- in LDS when rr <= 128 (132 KiB), else in a
+//                     global scratch -- and each round updates every 2 x 2
+//                     pair block B(a, b) <- M_a^H B M_b once (rows and columns
+//                     together, each element read and written once).  The
+//                     rotations are logged; the eigenvalues come out ascending
+//                     with the rank of every column.
+//   k_jacobi_vectors  rows of V are independent under column rotations, so V
+//                     is rebuilt from the log by many workgroups, 8 rows each,
+//                     held in LDS; the log is staged 8 rounds at a time.
+#include "device_common.h"
+
+namespace aosx {
+
+constexpr int EIGH_LDS_MAX = 128;  // rr up to which A lives in LDS
+constexpr int EIGH_MAX = 256;
+constexpr int EIGH_SWEEPS = 40;
+
+struct Rot {
+    double c, s, er, ei;  // e^{i phi} = er + i ei
+};
+
+__device__ __forceinline__ void rr_pair(int rr, int t, int a, int &p, int &q) {
+    if (a == 0) {
+        p = rr - 1;
+        q = t;
+    } else {
+        p = (t + a) % (rr - 1);
+        q = (t - a + (rr - 1)) % (rr - 1);
+    }
+    if (p > q) {
+        const int x = p;
+        p = q;
+        q = x;
+    }
+}
+
+__device__ __forceinline__ int packed(int rr, int i, int j) {  // i <= j
+    return i * rr - ((i * (i - 1)) >> 1) + (j - i);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <bool IN_LDS>
+__global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ A_all, int ld, int64_t stride,
+                                                        const int32_t *__restrict__ rk, c64 *__restrict__ scratch,
+                                                        Rot *__restrict__ log, int64_t log_stride,
+                                                        int32_t *__restrict__ rounds_out, double *__restrict__ w_all,
+                                                        int32_t *__restrict__ rank_all, int max_sweeps, double tol) {
+    __shared__ c64 Sl[IN_LDS ? EIGH_LDS_MAX * (EIGH_LDS_MAX + 1) / 2 : 1];
+    __shared__ uint8_t blk_a[IN_LDS ? 2080 : 8256], blk_b[IN_LDS ? 2080 : 8256];
+    __shared__ Rot rot[EIGH_MAX / 2];
+    __shared__ int16_t rp[EIGH_MAX / 2], rq[EIGH_MAX / 2];
+    __shared__ double part[2][16];
+    __shared__ int done;
+    const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    const int r = rk[k];
+    const int rr = r + (r & 1), npairs = rr / 2, nrounds = rr - 1;
+    const int nblocks = npairs * (npairs + 1) / 2;
+    c64 *S = IN_LDS ? Sl : scratch + (int64_t)k * (EIGH_MAX * (EIGH_MAX + 1) / 2);
+    const c64 *A = A_all + k * stride;
+    Rot *lg = log + (int64_t)k * log_stride;
+    // packed upper triangle; the padding index (odd r) is a zero row/column
+    for (int e = tid; e < rr * rr; e += nthr) {
+        const int i = e / rr, j = e % rr;
+        if (j < i) continue;
+        S[packed(rr, i, j)] = (i < r && j < r) ? A[(int64_t)i * ld + j] : c64{0.0, 0.0};
+    }
+    for (int a = tid; a < npairs; a += nthr) {
+        const int base = a * npairs - ((a * (a - 1)) >> 1);
+        for (int b = a; b < npairs; ++b) {
+            blk_a[base + b - a] = (uint8_t)a;
+            blk_b[base + b - a] = (uint8_t)b;
+        }
+    }
+    __syncthreads();
+    auto at = [&](int i, int j) -> c64 { return i <= j ? S[packed(rr, i, j)] : cconj(S[packed(rr, j, i)]); };
+    int sweep = 0;
+    for (; sweep < max_sweeps && r > 1; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < rr * rr; e += nthr) {
+            const int i = e / rr, j = e % rr;
+            if (j < i) continue;
+            const double v = cabs2(S[packed(rr, i, j)]);
+            tot += i == j ? v : 2.0 * v;
+            off += i == j ? 0.0 : 2.0 * v;
+        }
+        off = wave_sum(off);
+        tot = wave_sum(tot);
+        if ((tid & 63) == 0) {
+            part[0][tid >> 6] = off;
+            part[1][tid >> 6] = tot;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double o = 0.0, s = 0.0;
+            for (int q = 0; q < nthr / 64; ++q) {
+                o += part[0][q];
+                s += part[1][q];
+            }
+            done = !(o > tol * tol * s);
+        }
+        __syncthreads();
+        if (done) break;
+        for (int t = 0; t < nrounds; ++t) {
+            if (tid < npairs) {
+                int p, q;
+                rr_pair(rr, t, tid, p, q);
+                Rot g = {1.0, 0.0, 1.0, 0.0};
+                if (q < r) {
+                    const c64 b = S[packed(rr, p, q)];
+                    const double mb = sqrt(cabs2(b));
+                    const double app = S[packed(rr, p, p)].re, aqq = S[packed(rr, q, q)].re;
+                    if (mb > 1e-300 && mb > 1e-18 * sqrt(fabs(app * aqq))) {
+                        const double tau = (aqq - app) / (2.0 * mb);
+                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        g.c = 1.0 / sqrt(1.0 + tt * tt);
+                        g.s = tt * g.c;
+                        g.er = b.re / mb;
+                        g.ei = b.im / mb;
+                    }
+                }
+                rot[tid] = g;
+                rp[tid] = (int16_t)p;
+                rq[tid] = (int16_t)q;
+                lg[((int64_t)sweep * nrounds + t) * npairs + tid] = g;
+            }
+            __syncthreads();
+            for (int e = tid; e < nblocks; e += nthr) {
+                const int a = blk_a[e], b = blk_b[e];
+                const Rot ga = rot[a], gb = rot[b];
+                if (ga.s == 0.0 && gb.s == 0.0) continue;
+                const int pa = rp[a], qa = rq[a], pb = rp[b], qb = rq[b];
+                // B = [[A(pa,pb), A(pa,qb)], [A(qa,pb), A(qa,qb)]]
+                c64 b00 = at(pa, pb), b01 = at(pa, qb), b10 = at(qa, pb), b11 = at(qa, qb);
+                // B M_b: col0 = c B0 - s e^{-i phi} B1, col1 = s B0 + c e^{-i phi} B1
+                const c64 eb = {gb.er, -gb.ei};
+                c64 x0 = cmul(eb, b01), x1 = cmul(eb, b11);
+                c64 n00 = {gb.c * b00.re - gb.s * x0.re, gb.c * b00.im - gb.s * x0.im};
+                c64 n01 = {gb.s * b00.re + gb.c * x0.re, gb.s * b00.im + gb.c * x0.im};
+                c64 n10 = {gb.c * b10.re - gb.s * x1.re, gb.c * b10.im - gb.s * x1.im};
+                c64 n11 = {gb.s * b10.re + gb.c * x1.re, gb.s * b10.im + gb.c * x1.im};
+                // M_a^H X: row0 = c X0 - s e^{i phi} X1, row1 = s X0 + c e^{i phi} X1
+                const c64 ea = {ga.er, ga.ei};
+                const c64 y0 = cmul(ea, n10), y1 = cmul(ea, n11);
+                b00 = {ga.c * n00.re - ga.s * y0.re, ga.c * n00.im - ga.s * y0.im};
+                b01 = {ga.c * n01.re - ga.s * y1.re, ga.c * n01.im - ga.s * y1.im};
+                b10 = {ga.s * n00.re + ga.c * y0.re, ga.s * n00.im + ga.c * y0.im};
+                b11 = {ga.s * n01.re + ga.c * y1.re, ga.s * n01.im + ga.c * y1.im};
+                if (a == b) {  // the diagonal block: (p,p), (p,q), (q,q)
+                    S[packed(rr, pa, pa)] = {b00.re, 0.0};
+                    S[packed(rr, pa, qa)] = b01;
+                    S[packed(rr, qa, qa)] = {b11.re, 0.0};
+                } else {
+                    auto put = [&](int i, int j, c64 v) {
+                        if (i <= j) S[packed(rr, i, j)] = v;
+                        else S[packed(rr, j, i)] = cconj(v);
+                    };
+                    put(pa, pb, b00);
+                    put(pa, qb, b01);
+                    put(qa, pb, b10);
+                    put(qa, qb, b11);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) rounds_out[k] = sweep * nrounds;
+    // eigenvalues ascending by rank counting (stable); rank of every column
+    double *w = w_all + (int64_t)k * ld;
+    for (int i = tid; i < r; i += nthr) {
+        const double wi = S[packed(rr, i, i)].re;
+        int rank = 0;
+        for (int j = 0; j < r; ++j) {
+            const double wj = S[packed(rr, j, j)].re;
+            rank += (wj < wi) || (wj == wi && j < i);
+        }
+        w[rank] = wi;
+        rank_all[(int64_t)k * EIGH_MAX + i] = rank;
+    }
+}
+
+constexpr int EIGH_ROWS = 8;    // rows of V per workgroup
+constexpr int EIGH_BATCH = 8;   // logged rounds staged per batch
+
+// Vout[k][row][rank[j]] = V[row][j] for rows/columns < r, V = product of the
+// logged rotations applied to the identity.
+__global__ void __launch_bounds__(256) k_jacobi_vectors(c64 *__restrict__ Vout_all, int ld, int64_t stride,
+                                                        const int32_t *__restrict__ rk,
+                                                        const Rot *__restrict__ log, int64_t log_stride,
+                                                        const int32_t *__restrict__ rounds_all,
+                                                        const int32_t *__restrict__ rank_all) {
+    __shared__ c64 V[EIGH_ROWS][EIGH_MAX];
+    __shared__ Rot R[EIGH_BATCH][EIGH_MAX / 2];
+    const int k = blockIdx.y, tid = threadIdx.x;
+    const int r = rk[k];
+    const int row0 = blockIdx.x * EIGH_ROWS;
+    if (row0 >= r) return;
+    const int rr = r + (r & 1), npairs = rr / 2, nrounds = rr - 1;
+    const int rounds = rounds_all[k];
+    const Rot *lg = log + (int64_t)k * log_stride;
+    for (int e = tid; e < EIGH_ROWS * rr; e += 256) {
+        const int lr = e / rr, j = e % rr;
+        V[lr][j] = {row0 + lr == j ? 1.0 : 0.0, 0.0};
+    }
+    const int items = EIGH_ROWS * npairs;
+    for (int t0 = 0; t0 < rounds; t0 += EIGH_BATCH) {
+        const int nb = min(EIGH_BATCH, rounds - t0);
+        __syncthreads();  // the previous batch is applied
+        for (int e = tid; e < nb * npairs; e += 256) R[e / npairs][e % npairs] = lg[(int64_t)t0 * npairs + e];
+        __syncthreads();
+        for (int u = 0; u < nb; ++u) {
+            const int t = (t0 + u) % nrounds;
+            for (int e = tid; e < items; e += 256) {
+                const int lr = e / npairs, a = e % npairs;
+                const Rot g = R[u][a];
+                if (g.s == 0.0) continue;
+                int p, q;
+                rr_pair(rr, t, a, p, q);
+                const c64 em = {g.er, -g.ei};
+                const c64 x = V[lr][p], y = cmul(em, V[lr][q]);
+                V[lr][p] = {g.c * x.re - g.s * y.re, g.c * x.im - g.s * y.im};
+                V[lr][q] = {g.s * x.re + g.c * y.re, g.s * x.im + g.c * y.im};
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    c64 *Vout = Vout_all + k * stride;
+    const int32_t *rank = rank_all + (int64_t)k * EIGH_MAX;
+    for (int e = tid; e < EIGH_ROWS * r; e += 256) {
+        const int lr = e / r, j = e % r;
+        if (row0 + lr < r) Vout[(int64_t)(row0 + lr) * ld + rank[j]] = V[lr][j];
+    }
+}
+
+}  // namespace aosx

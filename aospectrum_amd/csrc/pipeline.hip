@@ -24,6 +24,7 @@
 #include "bundle.h"
 #include "kernels_asm.hip"
 #include "kernels_dense.hip"
+#include "kernels_eigh.hip"
 #include "kernels_lu.hip"
 #include "kernels_solve.hip"
 
@@ -459,7 +460,7 @@ int Bundle::autotune(int nmat, int precision, const std::vector<int> &candidates
     const size_t ff = factor_floats(), dif = (size_t)ts.n_diag * 2 * TILE_FLOATS;
     float *tiles = buf<float>("tiles", (size_t)nmat * ff);
     float *dinv = buf<float>("dinv", (size_t)nmat * dif);
-    int *repl = buf<int>("replaced", 65536);
+    int *repl = buf<int>("replaced", (size_t)std::max(nmat, 256));
     // shifts on the imaginary axis: the factorisation's cost does not depend
     // on the values (static pivoting, fixed structure)
     std::vector<int32_t> kidx(nmat, 0);
@@ -789,6 +790,36 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     }
 }
 
+// -------------------------------------------------------- eigenpair residual
+void Bundle::eig_residual(int64_t nk, const double *kpts, const float2 *h_in, const float2 *s_in, int count,
+                          const double *energies, const double *vectors, double *norm) {
+    AOSX_HIP(hipSetDevice(device));
+    float2 *d_H = buf<float2>("H", (size_t)nk * nnz);
+    float2 *d_S = buf<float2>("S", (size_t)nk * nnz);
+    if (h_in) {
+        AOSX_HIP(hipMemcpyAsync(d_H, h_in, (size_t)nk * nnz * 8, hipMemcpyHostToDevice, stream));
+        AOSX_HIP(hipMemcpyAsync(d_S, s_in, (size_t)nk * nnz * 8, hipMemcpyHostToDevice, stream));
+    } else {
+        assemble(kpts, nk, d_H, d_S);
+    }
+    double2 *d_vec = buf<double2>("r_vec", (size_t)nk * n * count);
+    double *d_e = buf<double>("r_energy", (size_t)nk * count);
+    int32_t *d_cnt = buf<int32_t>("r_cnt", nk);
+    double *d_n2 = buf<double>("r_norm2", (size_t)nk * count + 2);
+    std::vector<int32_t> cnt(nk, count);
+    AOSX_HIP(hipMemcpyAsync(d_vec, vectors, (size_t)nk * n * count * sizeof(double2), hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_e, energies, (size_t)nk * count * 8, hipMemcpyHostToDevice, stream));
+    AOSX_HIP(hipMemcpyAsync(d_cnt, cnt.data(), nk * 4, hipMemcpyHostToDevice, stream));
+    zero(d_n2, ((size_t)nk * count * 8 + 15) / 16 * 16);
+    k_eig_residual<<<dim3(blocks_for(n, 4, 1024), (unsigned)nk), 256, 0, stream>>>(d_n2, d_vec, count, d_cnt, d_e, d_H,
+                                                                                  d_S, d_indptr, d_indices, n, nnz);
+    AOSX_LAUNCHED();
+    std::vector<double> n2((size_t)nk * count);
+    AOSX_HIP(hipMemcpyAsync(n2.data(), d_n2, n2.size() * 8, hipMemcpyDeviceToHost, stream));
+    AOSX_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n2.size(); ++i) norm[i] = std::sqrt(n2[i]);
+}
+
 // ---------------------------------------------------------------- CISS band
 namespace {
 struct Subspace {
@@ -797,12 +828,40 @@ struct Subspace {
     std::vector<std::vector<float>> energies;
     std::vector<double> residual;
     float2 *ritz = nullptr;       // [nk][n][rmax] Ritz block Q C (when requested)
+    std::vector<std::vector<float2>> vectors;  // selected Ritz vectors [n][count] (when requested)
 };
 }  // namespace
 
+// Hermitian eigendecomposition of nk problems of order rk[k] <= rmax at A
+// (leading dimension ld, `stride` elements apart): ascending eigenvalues into
+// w[k][*], eigenvectors as the columns of Vout (rows and columns < rk[k]).
+// A is left unchanged.
+static void eigh_batched(Bundle &B, const c64 *A, int ld, int64_t stride, const int32_t *d_rk, int rmax, int nk,
+                         c64 *Vout, double *w) {
+    if (rmax > EIGH_MAX) throw Error(AOSX_SEARCH_DIMENSION, "projected problem exceeds 256 columns");
+    hipStream_t st = B.stream;
+    const int rr = rmax + (rmax & 1);
+    const int64_t log_stride = (int64_t)EIGH_SWEEPS * std::max(1, rr - 1) * (rr / 2);
+    Rot *log = B.buf<Rot>("e_log", (size_t)nk * log_stride);
+    int32_t *rounds = B.buf<int32_t>("e_rounds", nk);
+    int32_t *rank = B.buf<int32_t>("e_rank", (size_t)nk * EIGH_MAX);
+    if (rr <= EIGH_LDS_MAX) {
+        k_jacobi_rotate<true><<<nk, 1024, 0, st>>>(A, ld, stride, d_rk, nullptr, log, log_stride, rounds, w, rank,
+                                                   EIGH_SWEEPS, 1e-15);
+    } else {
+        c64 *scratch = B.buf<c64>("e_packed", (size_t)nk * (EIGH_MAX * (EIGH_MAX + 1) / 2));
+        k_jacobi_rotate<false><<<nk, 1024, 0, st>>>(A, ld, stride, d_rk, scratch, log, log_stride, rounds, w, rank,
+                                                    EIGH_SWEEPS, 1e-15);
+    }
+    AOSX_LAUNCHED();
+    k_jacobi_vectors<<<dim3((rmax + EIGH_ROWS - 1) / EIGH_ROWS, nk), 256, 0, st>>>(Vout, ld, stride, d_rk, log,
+                                                                                    log_stride, rounds, rank);
+    AOSX_LAUNCHED();
+}
+
 static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, const float2 *d_H,
                                  const float2 *d_S, const CissParams &p, const double *lower, const double *upper,
-                                 Subspace &out, double *stage, const char *ritz_name) {
+                                 Subspace &out, double *stage, const char *ritz_name, bool want_vectors = false) {
     const int64_t n = B.n, nnz = B.nnz;
     hipStream_t st = B.stream;
     double t0 = now();
@@ -827,14 +886,12 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     c64 *d_G = B.buf<c64>("c_G", (size_t)nk * P * P);
     gram(d_G, P, d_M, d_SM, P, (int64_t)n * P, 1);
     // eigh(G)
-    c64 *d_V = B.buf<c64>("c_V", (size_t)nk * P * P);
     c64 *d_U = B.buf<c64>("c_U", (size_t)nk * P * P);
     double *d_w = B.buf<double>("c_w", (size_t)nk * P);
     std::vector<int32_t> rP(nk, P);
     int32_t *d_rP = B.buf<int32_t>("c_rP", nk);
     AOSX_HIP(hipMemcpyAsync(d_rP, rP.data(), nk * 4, hipMemcpyHostToDevice, st));
-    k_jacobi<<<nk, 1024, 0, st>>>(d_G, d_V, d_U, d_w, d_rP, P, (int64_t)P * P, 40, 1e-15, nullptr);
-    AOSX_LAUNCHED();
+    eigh_batched(B, d_G, P, (int64_t)P * P, d_rP, P, nk, d_U, d_w);
     std::vector<double> w((size_t)nk * P);
     AOSX_HIP(hipMemcpyAsync(w.data(), d_w, w.size() * 8, hipMemcpyDeviceToHost, st));
     AOSX_HIP(hipStreamSynchronize(st));
@@ -932,12 +989,10 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     k_small_gemm<<<sg, 64, 0, st>>>(d_Wm, d_Tm, 0, d_Li, 1, d_rk, R, RR);
     AOSX_LAUNCHED();
     c64 *d_Y = B.buf<c64>("c_Y", (size_t)nk * RR);
-    c64 *d_Yw = B.buf<c64>("c_Yw", (size_t)nk * RR);
     double *d_E = B.buf<double>("c_E", (size_t)nk * R);
     k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Y), nk * RR);
     AOSX_LAUNCHED();
-    k_jacobi<<<nk, 1024, 0, st>>>(d_Wm, d_Yw, d_Y, d_E, d_rk, R, RR, 40, 1e-15, nullptr);
-    AOSX_LAUNCHED();
+    eigh_batched(B, d_Wm, R, RR, d_rk, rmax, nk, d_Y, d_E);
     k_small_gemm<<<sg, 64, 0, st>>>(d_C, d_Li, 1, d_Y, 0, d_rk, R, RR);
     AOSX_LAUNCHED();
     double *d_score = B.buf<double>("c_score", (size_t)nk * R);
@@ -1011,10 +1066,30 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
         AOSX_LAUNCHED();
         std::vector<double> n2((size_t)nk * Mc);
         AOSX_HIP(hipMemcpyAsync(n2.data(), d_n2, n2.size() * 8, hipMemcpyDeviceToHost, st));
+        std::vector<double2> hv;
+        if (want_vectors) {
+            hv.resize((size_t)nk * n * Mc);
+            AOSX_HIP(hipMemcpyAsync(hv.data(), d_vec, hv.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        }
         AOSX_HIP(hipStreamSynchronize(st));
         for (int k = 0; k < nk; ++k)
             for (int j = 0; j < scnt[k]; ++j)
                 out.residual[k] = std::max(out.residual[k], std::sqrt(n2[(size_t)k * Mc + j]));
+        if (want_vectors) {
+            out.vectors.assign(nk, {});
+            for (int k = 0; k < nk; ++k) {
+                const int c = scnt[k];
+                std::vector<float2> &v = out.vectors[k];
+                v.resize((size_t)n * c);
+                for (int64_t row = 0; row < n; ++row)
+                    for (int j = 0; j < c; ++j) {
+                        const double2 x = hv[((size_t)k * n + row) * Mc + j];
+                        v[(size_t)row * c + j] = make_float2((float)x.x, (float)x.y);
+                    }
+            }
+        }
+    } else if (want_vectors) {
+        out.vectors.assign(nk, {});
     }
     stage[8] += now() - t2;
 }
@@ -1083,6 +1158,9 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     out.max_residual.assign(nk_total, 0.0);
     out.rank.assign(nk_total, 0);
     out.saturated.assign(nk_total, 0);
+    out.pass_rank.assign(nk_total, 0);
+    out.pass_width.assign(nk_total, 0);
+    if (p.want_vectors) out.vectors.assign(nk_total, {});
     const int K = std::max(1, p.k_batch);
     const int nfac = p.conjugate_pairs ? N / 2 : N;
     const int cb = std::max(1, std::min(p.contour_batch_size, nfac));
@@ -1092,7 +1170,6 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
     // intermediate passes only need a subspace, not converged solves -- one
     // step there, the full count on the last filter application
     const int ir_first = p.refinement_passes > 0 ? std::min(ir_steps, 1) : ir_steps;
-    int *d_repl = buf<int>("replaced", 65536);
     // AOSX_PROGRESS: one line per factorisation group and pass on stderr
     const bool progress = std::getenv("AOSX_PROGRESS") != nullptr;
     auto report = [&](int64_t kb, int pass, int g, int ng) {
@@ -1190,9 +1267,9 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 AOSX_LAUNCHED();
             }
         };
-        const int64_t repl_zero4 = 65536 * 4 / 16;
-        k_fill_zero<<<64, 256, 0, stream>>>(reinterpret_cast<float4 *>(d_repl), repl_zero4);
-        AOSX_LAUNCHED();
+        // pivot-replacement counters, one per resident factorisation slot
+        int *d_repl = buf<int>("replaced", (size_t)std::max(slots, 256));
+        zero(d_repl, (size_t)(slots + 3) / 4 * 16);
         for (int g = 0; g < ngroups; ++g) {
             Group &gr = groups[g];
             const int gsz = (int)gr.kidx.size();
@@ -1228,8 +1305,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             khi[k] = contours[kb + k].upper;
         }
         compress_and_project(*this, Kc, d_M, P, d_H, d_S, p, klo.data(), khi.data(), sub, out.stages,
-                             p.refinement_passes > 0 ? ritz_names[0] : nullptr);
+                             p.refinement_passes > 0 ? ritz_names[0] : nullptr,
+                             p.want_vectors && p.refinement_passes == 0);
         std::vector<int> rank0 = sub.rank;
+        std::vector<int> prank(Kc, 0), pwidth(Kc, 0);
         // filter passes: the block [Ritz vectors | augmentation] is filtered
         // again through the resident factorisations.  The augmentation starts
         // as a second independent probe block and is itself re-filtered every
@@ -1301,7 +1380,16 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             const bool more = pass + 1 < p.refinement_passes;
             Subspace next;
             compress_and_project(*this, Kc, d_Y, W2, d_H, d_S, p, klo.data(), khi.data(), next, out.stages,
-                                 more ? ritz_names[(pass + 1) & 1] : nullptr);
+                                 more ? ritz_names[(pass + 1) & 1] : nullptr, p.want_vectors && !more);
+            // Capacity of the filtered space: the moment-pass directions plus
+            // the fresh augmentation of the first pass.  Later passes add no
+            // direction (their augmentation is the filtered first one, which
+            // already lies in the previous pass's span), so every pass's rank
+            // is at most rank0 + A; a pass that reaches it may miss states.
+            for (int k = 0; k < Kc; ++k) {
+                if (pass == 0) pwidth[k] = rank0[k] + (augment ? A : 0);
+                prank[k] = next.rank[k];
+            }
             if (std::getenv("AOSX_DEBUG"))
                 for (int k = 0; k < Kc; ++k)
                     std::fprintf(stderr, "AOSX_DEBUG|pass %d|k=%d|W=%d|rank=%d|count=%zu|residual=%.3e\n", pass + 1, k,
@@ -1312,7 +1400,13 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             out.energies[kb + k] = sub.energies[k];
             out.max_residual[kb + k] = sub.residual[k];
             out.rank[kb + k] = rank0[k];
-            out.saturated[kb + k] = rank0[k] == L * Ms;
+            out.pass_rank[kb + k] = prank[k];
+            out.pass_width[kb + k] = pwidth[k];
+            // the moment space is full, or the last filter pass kept every
+            // column of its block: more states may lie in the window than the
+            // search space holds (ciss.py:342)
+            out.saturated[kb + k] = rank0[k] == L * Ms || (p.refinement_passes > 0 && prank[k] >= pwidth[k]);
+            if (p.want_vectors) out.vectors[kb + k] = std::move(sub.vectors[k]);
         }
     }
     out.stages[11] += now() - t_start;

@@ -25,6 +25,8 @@
  *   aosx_band_solve_v1      -- CudssCissSolver.solve_interval (src/aospectrum/ciss.py:266-391)
  *                              applied to BlochWorkspace.update for each k point
  *                              (src/aospectrum/band.py:227-253), for a batch of k.
+ *   aosx_eig_residual_v1    -- the eigenpair residual of CudssCissSolver.solve_interval
+ *                              (src/aospectrum/ciss.py:362-372) for given pairs.
  *   aosx_bundle_destroy_v1  -- CudssShiftSession.close (bindings.cpp:103).
  */
 #ifndef AOSX_H
@@ -258,7 +260,17 @@ typedef struct {
     int32_t *energy_counts;        /* [n_kpoints]                                  */
     double *maximum_residual_ev;   /* [n_kpoints]                                  */
     int32_t *subspace_rank;        /* [n_kpoints]                                  */
-    int32_t *saturated;            /* [n_kpoints], 1 when rank == L*M              */
+    int32_t *saturated;            /* [n_kpoints], 1 when the search space may be  */
+                                   /* too small: moment rank == L*M, or the last   */
+                                   /* filter pass reached the pass capacity        */
+    aosx_complex64 *eigenvectors;  /* [n_kpoints][n][energy_capacity] out, or NULL: */
+                                   /* the selected Ritz vectors (S-normalised,     */
+                                   /* orbital order), column j for energies_ev[j]  */
+    int32_t *pass_rank;            /* [n_kpoints] out, or NULL: numerical rank of   */
+                                   /* the last filter pass's block                 */
+    int32_t *pass_width;           /* [n_kpoints] out, or NULL: the pass capacity,  */
+                                   /* moment rank + augmentation_width (the later  */
+                                   /* passes refilter the same span)               */
 } aosx_band_request_v1;
 
 #define AOSX_STAGE_COUNT 12
@@ -279,6 +291,27 @@ typedef struct {
 
 int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *request,
                        aosx_band_result_v1 *result);
+
+/* ------------------------------------------------------- eigenpair residual
+ * ||H(k) v_j - e_j S(k) v_j||_2 for caller-given pairs, in FP64 from the
+ * complex64 H(k), S(k) of the bundle: the residual of
+ * src/aospectrum/ciss.py:362-372 that aosx_band_solve_v1 reports as
+ * maximum_residual_ev, exposed so that it can be checked independently. */
+typedef struct {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    int64_t n_kpoints;
+    const double *k_fractional;    /* [n_kpoints][3]; NULL when h_data/s_data given */
+    const aosx_complex64 *h_data;  /* [n_kpoints][nnz] explicit matrices, or NULL  */
+    const aosx_complex64 *s_data;
+    int32_t count;                 /* pairs per k point, 1..256                    */
+    int32_t reserved;
+    const double *energies_ev;     /* [n_kpoints][count]                           */
+    const double *vectors;         /* [n_kpoints][n][count] complex128 (re, im)    */
+    double *residual_ev;           /* [n_kpoints][count] out                       */
+} aosx_residual_request_v1;
+
+int aosx_eig_residual_v1(aosx_bundle *bundle, const aosx_residual_request_v1 *request, aosx_status_v1 *result);
 
 /* ------------------------------------------------------- kernel statistics
  * Device-event timings of the hot kernels, accumulated while enabled.  No
@@ -344,8 +377,11 @@ typedef struct {
 int aosx_bundle_autotune_v1(aosx_bundle *bundle, const aosx_autotune_request_v1 *request,
                             aosx_autotune_result_v1 *result);
 
-/* sizeof of every request/result struct, in declaration order, so bindings
- * can verify their layouts; returns the number of sizes written. */
+/* sizeof of every request/result struct, so bindings can verify their
+ * layouts, in this order: bundle request, bundle result, status, assemble
+ * request, shift request, shift result, band request, band result, kernel
+ * stats, autotune request, autotune result, residual request; returns the
+ * number of sizes written. */
 int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity);
 
 #ifdef __cplusplus
