@@ -120,7 +120,7 @@ int aosx_bundle_create_v1(const aosx_bundle_request_v1 *q, aosx_bundle_result_v1
             return fail(res, AOSX_NONFINITE_VALUES, 0, "operator blocks contain non-finite values");
         }
         double t1 = now();
-        std::vector<int32_t> order;
+        std::vector<int32_t> order, groups;
         int64_t atom_fill = 0;
         const bool use_nd = q->ordering == AOSX_ORDERING_NESTED_DISSECTION ||
                             (q->ordering == AOSX_ORDERING_AUTO && q->atom_positions);
@@ -128,15 +128,23 @@ int aosx_bundle_create_v1(const aosx_bundle_request_v1 *q, aosx_bundle_result_v1
             for (int64_t i = 0; i < 3 * q->n_atoms; ++i)
                 if (!std::isfinite(q->atom_positions[i]))
                     return fail(res, AOSX_NONFINITE_VALUES, 0, "atom positions are not finite");
-            why = aosx::order_atoms_nd(bt, q->atom_positions, order);
+            why = aosx::order_atoms_nd(bt, q->atom_positions, order, groups);
             if (why.empty()) atom_fill = aosx::atom_fill(bt, order);
         } else {
             why = aosx::order_atoms(bt, q->ordering == AOSX_ORDERING_NATURAL ? 1 : 0, order, atom_fill);
         }
         if (!why.empty()) return fail(res, AOSX_INVALID_BLOCKS, 0, "%s", why.c_str());
         double t2 = now();
-        why = aosx::build_tiles(bt, b->pat, order, 64, b->ts);
+        why = aosx::build_tiles(bt, b->pat, order, groups, 64, b->ts);
         if (!why.empty()) return fail(res, AOSX_INT32_OVERFLOW, 0, "%s", why.c_str());
+        if (!groups.empty()) {
+            // node-aligned tiles pay off when the nodes span several tiles;
+            // on small bundles the padding can cost more than it saves
+            aosx::TileStructure packed;
+            why = aosx::build_tiles(bt, b->pat, order, {}, 64, packed);
+            if (!why.empty()) return fail(res, AOSX_INT32_OVERFLOW, 0, "%s", why.c_str());
+            if (packed.factor_flops < b->ts.factor_flops) b->ts = std::move(packed);
+        }
         b->ts.atom_fill_entries = atom_fill;
         double t3 = now();
         b->n = q->n_orbitals;
@@ -181,7 +189,11 @@ int aosx_bundle_pattern_v1(const aosx_bundle *bundle, int32_t *indptr, int32_t *
     const Bundle &b = *bundle->impl;
     if (indptr) std::memcpy(indptr, b.pat.indptr.data(), b.pat.indptr.size() * 4);
     if (indices) std::memcpy(indices, b.pat.indices.data(), b.pat.indices.size() * 4);
-    if (permutation) std::memcpy(permutation, b.ts.perm.data(), b.ts.perm.size() * 4);
+    if (permutation) {  // the orbitals in elimination order (padding rows skipped)
+        int64_t q = 0;
+        for (int32_t v : b.ts.perm)
+            if (v >= 0) permutation[q++] = v;
+    }
     return 0;
 }
 

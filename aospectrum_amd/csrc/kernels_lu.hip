@@ -38,13 +38,14 @@ __global__ void k_fill_zero(float4 *__restrict__ p, int64_t count4) {
 }
 
 // A_m = H_k - z_m S_k scattered into the tile storage of matrix m
-// (matrix m uses k point kidx[m] and shift z[m]); padded rows get a unit
-// diagonal so the trailing tile stays nonsingular.
+// (matrix m uses k point kidx[m] and shift z[m]); padding rows get a unit
+// diagonal so their tiles stay nonsingular.
 __global__ void k_scatter_shifted(float *__restrict__ tiles, int64_t mat_stride, const float2 *__restrict__ h,
                                   const float2 *__restrict__ s, int64_t nnz, const int32_t *__restrict__ kidx,
                                   const float2 *__restrict__ z, const int32_t *__restrict__ rows,
                                   const int32_t *__restrict__ cols, const int32_t *__restrict__ iperm,
-                                  const int32_t *__restrict__ tile_id, int nt, int64_t n) {
+                                  const int32_t *__restrict__ tile_id, int nt,
+                                  const int32_t *__restrict__ padding_rows, int n_padding) {
     const int m = blockIdx.y;
     const float2 zm = z[m];
     const float2 *hk = h + (int64_t)kidx[m] * nnz;
@@ -63,8 +64,8 @@ __global__ void k_scatter_shifted(float *__restrict__ tiles, int64_t mat_stride,
         t[PLANE] = im;
     }
     if (blockIdx.x == 0) {
-        const int64_t padded = (int64_t)nt * TILE;
-        for (int64_t r = n + threadIdx.x; r < padded; r += blockDim.x) {
+        for (int q = threadIdx.x; q < n_padding; q += blockDim.x) {
+            const int32_t r = padding_rows[q];
             const int32_t id = tile_id[(int64_t)(r / TILE) * nt + r / TILE];
             base[(int64_t)id * TILE_FLOATS + (r % TILE) * TILE + r % TILE] = 1.0f;
         }

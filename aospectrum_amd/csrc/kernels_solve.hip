@@ -276,7 +276,8 @@ __global__ void k_pack_rhs(float *__restrict__ y, int64_t y_mat_stride, int nchu
         const int64_t row = e / W;
         const int col = (int)(e % W);
         float2 v = make_float2(0.f, 0.f);
-        if (row < n && col < w_src) v = s[(int64_t)perm[row] * w_src + col];
+        const int32_t orb = perm[row];  // -1 on padding rows
+        if (orb >= 0 && col < w_src) v = s[(int64_t)orb * w_src + col];
         float *blk = y + rhs_block(y_mat_stride, m, (int)(row / TILE), nchunk, col / TILE);
         const int off = (int)(row % TILE) * TILE + col % TILE;
         blk[off] = v.x;
@@ -284,18 +285,20 @@ __global__ void k_pack_rhs(float *__restrict__ y, int64_t y_mat_stride, int nchu
     }
 }
 
-// dst[m] (row-major n x w_dst, original order) <- Y[m]
+// dst[m] (row-major n x w_dst, original order) <- Y[m]; iperm maps an orbital
+// to its factor row
 __global__ void k_unpack(float2 *__restrict__ dst, int w_dst, const float *__restrict__ y, int64_t y_mat_stride,
-                         int nchunk, const int32_t *__restrict__ perm, int64_t n) {
+                         int nchunk, const int32_t *__restrict__ iperm, int64_t n) {
     const int m = blockIdx.y;
     float2 *d = dst + (int64_t)m * n * w_dst;
     const int64_t total = n * w_dst, stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-        const int64_t row = e / w_dst;
+        const int64_t orb = e / w_dst;
         const int col = (int)(e % w_dst);
+        const int64_t row = iperm[orb];
         const float *blk = y + rhs_block(y_mat_stride, m, (int)(row / TILE), nchunk, col / TILE);
         const int off = (int)(row % TILE) * TILE + col % TILE;
-        d[(int64_t)perm[row] * w_dst + col] = make_float2(blk[off], blk[PLANE + off]);
+        d[e] = make_float2(blk[off], blk[PLANE + off]);
     }
 }
 

@@ -185,7 +185,8 @@ Bundle::~Bundle() {
                         (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
                         (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
                         (void *)d_atom_off, (void *)d_pairs, (void *)d_quads, (void *)d_ppos, (void *)d_step_s,
-                        (void *)d_fwd_rows, (void *)d_bwd_rows, (void *)d_pair_mask, (void *)d_quad_mask})
+                        (void *)d_fwd_rows, (void *)d_bwd_rows, (void *)d_pair_mask, (void *)d_quad_mask,
+                        (void *)d_padding})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -276,6 +277,7 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     d_shifts = upload_vec(pat.shifts, resident_bytes);
     d_iperm = upload_vec(ts.iperm, resident_bytes);
     d_perm = upload_vec(ts.perm, resident_bytes);
+    d_padding = upload_vec(ts.padding_rows, resident_bytes);
     d_tile_id = upload_vec(ts.tile_id, resident_bytes);
     d_next = upload_vec(ts.next_idx, resident_bytes);
     d_prev = upload_vec(ts.prev_idx, resident_bytes);
@@ -350,15 +352,29 @@ void Bundle::plan_schur(int group) {
                             pairs.push_back(rows[b]);
                             pair_mask.push_back((int32_t)(steps_of(rows[a]) & steps_of(rows[b])));
                         }
-        // quads: 2 x 2 blocks of consecutive rows and columns, in 4 x 4 blocks
-        // of quads; -1 pads an odd row count
+        // quads: 2 x 2 blocks of rows and columns, in 4 x 4 blocks of quads; -1
+        // pads an odd row count.  The rows are paired in order of their step
+        // masks (the group's steps with an L tile in that row), so that the
+        // two rows -- and columns -- of a quad mostly share their steps: a
+        // quad's four waves then have work in the same steps, and few MFMAs
+        // run on the zeroed operands of a wave without an update.
+        std::vector<int32_t> qrows(rows);
+        std::vector<uint32_t> qmask(r);
+        for (int a = 0; a < r; ++a) qmask[a] = steps_of(rows[a]);
+        {
+            std::vector<int> idx(r);
+            for (int a = 0; a < r; ++a) idx[a] = a;
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return qmask[x] > qmask[y]; });
+            for (int a = 0; a < r; ++a) qrows[a] = rows[idx[a]];
+        }
+        auto qmarked = [&](int a, int b) { return a < r && b < r && mark[(size_t)qrows[a] * nt + qrows[b]] != 0; };
         for (int bi = 0; bi < r; bi += 8)
             for (int bj = 0; bj < r; bj += 8)
                 for (int a = bi; a < std::min(r, bi + 8); a += 2)
                     for (int b = bj; b < std::min(r, bj + 8); b += 2)
-                        if (marked(a, b) || marked(a, b + 1) || marked(a + 1, b) || marked(a + 1, b + 1)) {
-                            const int q4[4] = {rows[a], a + 1 < r ? rows[a + 1] : -1, rows[b],
-                                               b + 1 < r ? rows[b + 1] : -1};
+                        if (qmarked(a, b) || qmarked(a, b + 1) || qmarked(a + 1, b) || qmarked(a + 1, b + 1)) {
+                            const int q4[4] = {qrows[a], a + 1 < r ? qrows[a + 1] : -1, qrows[b],
+                                               b + 1 < r ? qrows[b + 1] : -1};
                             for (int e = 0; e < 4; ++e) {
                                 quads.push_back(q4[e]);
                                 quad_mask.push_back((int32_t)steps_of(q4[e]));
@@ -576,7 +592,8 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
         k_fill_zero<<<blocks_for(count4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(tiles), count4);
         AOSX_LAUNCHED();
         k_scatter_shifted<<<dim3(blocks_for(nnz, 256, 4096), nmat), 256, 0, stream>>>(
-            tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, n);
+            tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, d_padding,
+            (int)ts.padding_rows.size());
         AOSX_LAUNCHED();
         for (int g = 0; g < ng; ++g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
@@ -759,7 +776,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, d_rhs, W, d_ri, d_perm, n, padded);
     AOSX_LAUNCHED();
     solve(nmat, W, y2, tiles, dinv, ctrans, precision);
-    k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(x_out, W, y, ystride, nchunk, d_perm, n);
+    k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(x_out, W, y, ystride, nchunk, d_iperm, n);
     AOSX_LAUNCHED();
     if (ir_steps <= 0 && !rel_residual) return;
     float2 *r = buf<float2>("s_r", (size_t)nmat * n * W);
@@ -783,7 +800,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
         k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, r, W, d_id, d_perm, n, padded);
         AOSX_LAUNCHED();
         solve(nmat, W, y2, tiles, dinv, ctrans, precision);
-        k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(r, W, y, ystride, nchunk, d_perm, n);
+        k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(r, W, y, ystride, nchunk, d_iperm, n);
         AOSX_LAUNCHED();
         k_axpy<<<blocks_for((int64_t)nmat * n * W, 256, 8192), 256, 0, stream>>>(x_out, r, (int64_t)nmat * n * W);
         AOSX_LAUNCHED();
@@ -1325,17 +1342,31 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
         }
         for (int pass = 0; pass < p.refinement_passes; ++pass) {
             double tp = now();
-            const int R = sub.rmax;  // multiple of 32
-            const bool augment = d_aug && R + A <= 256;
-            const int W2 = augment ? R + A : R;
+            const int Rpad = sub.rmax;  // column stride of the Ritz block (multiple of 32)
+            int R = 0;                  // widest rank: the columns that can be nonzero
+            for (int k = 0; k < Kc; ++k) R = std::max(R, sub.rank[k]);
+            // The block is [Ritz vectors | augmentation].  The augmentation is
+            // trimmed (to no fewer than 32 columns) so that the block keeps to
+            // 128 columns when the rank allows: the solves then sweep two
+            // 64-column chunks instead of three and the projected problems stay
+            // within the LDS-resident eigensolver.
+            int Ause = 0;
+            if (d_aug) {
+                Ause = A;
+                if (R + A > 128 && R < 128) Ause = std::max(std::min(A, 32), 128 - R);
+                Ause = std::max(0, std::min(Ause, 256 - R));
+            }
+            const bool augment = Ause > 0;
+            const int W2 = R + Ause;
             const bool last_pass = pass + 1 == p.refinement_passes;
             const int ir_pass = last_pass ? ir_steps : std::min(ir_steps, 1);
             float2 *d_blk = buf<float2>("p_blk", (size_t)Kc * n * W2);
-            k_copy_cols<<<dim3(blocks_for(n * R, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, 0, sub.ritz, R, 0, R, n);
+            k_copy_cols<<<dim3(blocks_for(n * R, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, 0, sub.ritz, Rpad, 0, R,
+                                                                                   n);
             AOSX_LAUNCHED();
             if (augment) {
-                k_copy_cols<<<dim3(blocks_for(n * A, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, R, d_aug, A, 0, A,
-                                                                                       n);
+                k_copy_cols<<<dim3(blocks_for(n * Ause, 256, 4096), Kc), 256, 0, stream>>>(d_blk, W2, R, d_aug, A, 0,
+                                                                                          Ause, n);
                 AOSX_LAUNCHED();
             }
             float2 *d_rhs2 = buf<float2>("p_rhs", (size_t)Kc * n * W2);
@@ -1371,8 +1402,8 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 }
             }
             if (augment) {
-                k_copy_cols<<<dim3(blocks_for(n * A, 256, 4096), Kc), 256, 0, stream>>>(d_aug, A, 0, d_Y, W2, R, A,
-                                                                                       n);
+                k_copy_cols<<<dim3(blocks_for(n * Ause, 256, 4096), Kc), 256, 0, stream>>>(d_aug, A, 0, d_Y, W2, R,
+                                                                                          Ause, n);
                 AOSX_LAUNCHED();
             }
             AOSX_HIP(hipStreamSynchronize(stream));
@@ -1387,7 +1418,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             // already lies in the previous pass's span), so every pass's rank
             // is at most rank0 + A; a pass that reaches it may miss states.
             for (int k = 0; k < Kc; ++k) {
-                if (pass == 0) pwidth[k] = rank0[k] + (augment ? A : 0);
+                if (pass == 0) pwidth[k] = rank0[k] + Ause;
                 prank[k] = next.rank[k];
             }
             if (std::getenv("AOSX_DEBUG"))

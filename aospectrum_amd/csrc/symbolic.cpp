@@ -246,8 +246,14 @@ std::string order_atoms(const BlockTable &bt, int ordering, std::vector<int32_t>
 // edges that cross the split (greedy, highest remaining cut degree first), so
 // periodic-image edges -- which the block table carries -- are separated too.
 // Regions are ordered before their separator; regions of at most `leaf` atoms
-// keep coordinate order.
-std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<int32_t> &order) {
+// (or whose separator would take most of them) become leaves.  Inside every
+// node (leaf or separator) the atoms follow recursive coordinate bisection
+// (median of the widest axis, down to pairs), so that the consecutive atoms a
+// tile holds are spatial neighbours: a separator plane is cut into compact
+// patches instead of strips along its normal, and the tiles of a node couple
+// to few tiles of the nodes around it.
+std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<int32_t> &order,
+                           std::vector<int32_t> &groups) {
     const int64_t na = bt.n_atoms;
     std::vector<std::vector<int32_t>> adj(na);
     for (int64_t b = 0; b < bt.n_blocks; ++b) {
@@ -261,16 +267,40 @@ std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<
     std::vector<int32_t> side(na, -1);  // region-local marks: 0 A, 1 B, -1 outside
     std::vector<int32_t> cutdeg(na, 0);
     std::vector<char> covered(na, 0);
-    const size_t leaf = 12;
+    const size_t leaf = 32;
     order.clear();
     order.reserve(na);
+    groups.assign(1, 0);
+    std::function<void(int32_t *, int32_t *)> bisect = [&](int32_t *b, int32_t *e) {
+        if (e - b <= 2) return;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (const int32_t *p = b; p < e; ++p)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], pos[3 * *p + d]);
+                hi[d] = std::max(hi[d], pos[3 * *p + d]);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+        int32_t *mid = b + (e - b) / 2;
+        std::nth_element(b, mid, e, [&](int32_t x, int32_t y) {
+            return pos[3 * x + ax] < pos[3 * y + ax] || (pos[3 * x + ax] == pos[3 * y + ax] && x < y);
+        });
+        bisect(b, mid);
+        bisect(mid, e);
+    };
+    auto emit = [&](std::vector<int32_t> &node) {
+        bisect(node.data(), node.data() + node.size());
+        order.insert(order.end(), node.begin(), node.end());
+        groups.push_back((int32_t)order.size());
+    };
     struct Frame {
         std::vector<int32_t> set;
     };
     // explicit recursion: emit(region) = emit(A'), emit(B'), separator
     std::function<void(std::vector<int32_t> &)> nd = [&](std::vector<int32_t> &set) {
         if (set.size() <= leaf) {
-            order.insert(order.end(), set.begin(), set.end());
+            emit(set);
             return;
         }
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -316,14 +346,12 @@ std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<
         }
         for (int32_t a : sep) covered[a] = 0;
         if (A.empty() || B.empty() || sep.size() * 10 > set.size() * 9) {
-            order.insert(order.end(), set.begin(), set.end());
+            emit(set);
             return;
         }
-        std::stable_sort(sep.begin(), sep.end(),
-                         [&](int32_t x, int32_t y) { return pos[3 * x + axis] < pos[3 * y + axis]; });
         nd(A);
         nd(B);
-        order.insert(order.end(), sep.begin(), sep.end());
+        emit(sep);
     };
     std::vector<int32_t> all(na);
     std::iota(all.begin(), all.end(), 0);
@@ -363,7 +391,7 @@ int64_t atom_fill(const BlockTable &bt, const std::vector<int32_t> &order) {
 }
 
 std::string build_tiles(const BlockTable &bt, const Pattern &pat, const std::vector<int32_t> &atom_order,
-                        int tile, TileStructure &ts) {
+                        const std::vector<int32_t> &groups, int tile, TileStructure &ts) {
     const int64_t n = bt.n_orbitals, na = bt.n_atoms;
     ts.tile = tile;
     ts.n = n;
@@ -371,13 +399,25 @@ std::string build_tiles(const BlockTable &bt, const Pattern &pat, const std::vec
     ts.perm.clear();
     ts.perm.reserve(n);
     std::vector<int64_t> atom_start(na);
-    for (int32_t a : atom_order) {
+    std::vector<char> first(atom_order.size() + 1, 0);  // a group starts at this position
+    for (size_t g = 1; g + 1 < groups.size(); ++g) first[(size_t)groups[g]] = 1;
+    for (size_t q = 0; q < atom_order.size(); ++q) {
+        if (first[q])
+            while (ts.perm.size() % tile) ts.perm.push_back(-1);
+        const int32_t a = atom_order[q];
         atom_start[a] = (int64_t)ts.perm.size();
         for (int64_t r = bt.atom_offsets[a]; r < bt.atom_offsets[a + 1]; ++r) ts.perm.push_back((int32_t)r);
     }
+    while (ts.perm.size() % tile) ts.perm.push_back(-1);
+    ts.n_rows = (int64_t)ts.perm.size();
+    if (ts.n_rows > INT32_MAX) return "factor rows exceed int32";
     ts.iperm.assign(n, 0);
-    for (int64_t q = 0; q < n; ++q) ts.iperm[ts.perm[q]] = (int32_t)q;
-    const int nt = (int)((n + tile - 1) / tile);
+    ts.padding_rows.clear();
+    for (int64_t q = 0; q < ts.n_rows; ++q) {
+        if (ts.perm[q] >= 0) ts.iperm[ts.perm[q]] = (int32_t)q;
+        else ts.padding_rows.push_back((int32_t)q);
+    }
+    const int nt = (int)(ts.n_rows / tile);
     ts.n_diag = nt;
     Bitset g;
     g.resize(nt, nt);

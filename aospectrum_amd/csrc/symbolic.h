@@ -42,14 +42,19 @@ struct Pattern {
 // Tile-level static structure of L and U (symmetric, since no row exchanges
 // are made).  Orbitals are permuted atom by atom into elimination order and
 // cut into tiles of T rows; next[k] lists the tiles i > k with L(i,k) != 0,
-// which by symmetry are also the tiles j > k with U(k,j) != 0.
+// which by symmetry are also the tiles j > k with U(k,j) != 0.  When the
+// ordering comes in groups (the nodes of a nested dissection), every group
+// starts on a tile boundary: the rows between are padding rows with a unit
+// diagonal and no couplings, so that no tile mixes two groups.
 struct TileStructure {
     int tile = 64;
     int64_t n = 0;
+    int64_t n_rows = 0;                   // factor rows: Nt * T, padding included
     int n_diag = 0;                       // Nt
     std::vector<int32_t> atom_order;      // elimination position -> atom
-    std::vector<int32_t> perm;            // factor row -> orbital
+    std::vector<int32_t> perm;            // factor row -> orbital, -1 on padding rows
     std::vector<int32_t> iperm;           // orbital -> factor row
+    std::vector<int32_t> padding_rows;    // factor rows that hold no orbital
     std::vector<int32_t> next_ptr;        // [Nt + 1]
     std::vector<int32_t> next_idx;        // concatenated next[k], ascending
     std::vector<int32_t> prev_ptr;        // [Nt + 1]: tiles i < k with k in next[i]
@@ -66,9 +71,14 @@ struct TileStructure {
 std::string build_pattern(const BlockTable &blocks, Pattern &out, int &code);
 std::string order_atoms(const BlockTable &blocks, int ordering, std::vector<int32_t> &order,
                         int64_t &atom_fill_entries);
-std::string order_atoms_nd(const BlockTable &blocks, const double *positions, std::vector<int32_t> &order);
+// Nested dissection: `order` lists the atoms node by node (leaves and
+// separators in elimination order), node g holding order[groups[g] ..
+// groups[g+1]).
+std::string order_atoms_nd(const BlockTable &blocks, const double *positions, std::vector<int32_t> &order,
+                           std::vector<int32_t> &groups);
 int64_t atom_fill(const BlockTable &blocks, const std::vector<int32_t> &order);
-std::string build_tiles(const BlockTable &blocks, const Pattern &pattern,
-                        const std::vector<int32_t> &atom_order, int tile, TileStructure &out);
+// `groups` (may be empty) as from order_atoms_nd: each group starts a tile.
+std::string build_tiles(const BlockTable &blocks, const Pattern &pattern, const std::vector<int32_t> &atom_order,
+                        const std::vector<int32_t> &groups, int tile, TileStructure &out);
 
 }  // namespace aosx

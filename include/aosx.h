@@ -155,8 +155,9 @@ typedef struct {
 int aosx_bundle_create_v1(const aosx_bundle_request_v1 *request,
                           aosx_bundle_result_v1 *result, aosx_bundle **bundle);
 
-/* indptr [n+1], indices [nnz], permutation [n] (factor position -> orbital);
- * any pointer may be NULL. */
+/* indptr [n+1], indices [nnz], permutation [n] (the orbitals in elimination
+ * order; the factor also holds padding rows between nested-dissection nodes,
+ * which are not listed); any pointer may be NULL. */
 int aosx_bundle_pattern_v1(const aosx_bundle *bundle, int32_t *indptr, int32_t *indices,
                            int32_t *permutation, aosx_status_v1 *result);
 

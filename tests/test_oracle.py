@@ -92,13 +92,43 @@ def test_oracle_ciss_matches_fp64_eigensolver(small2d, small2d_pattern):
     assert res.energies_ev.size == ref.size == dense.inertia_count(H.toarray(), S.toarray(), lo, hi)
     assert np.max(np.abs(res.energies_ev - ref)) < 1e-4
     assert not res.saturated
-    # a plausible mistake -- the wrong sign of the moment weights -- breaks it
-    z, t, w = ciss.contour(lo, hi, ciss.Settings())
-    V = ciss.probes(H.shape[0], ciss.Settings())
-    sols = [ciss.superlu_solver(H, S, zj, np.asarray(S @ V, np.complex64)) for zj in z[:4]]
-    M_good = ciss.moments_from_solutions(sols, t[:4], w[:4], ciss.Settings(), H.shape[0])
-    M_bad = ciss.moments_from_solutions(sols, t[:4], -w[:4], ciss.Settings(), H.shape[0])
-    assert np.allclose(M_good, -M_bad)
+
+
+def test_moments_match_closed_form_filter_response():
+    """On a diagonal pencil H = diag(l), S = I with unit probes, the shifted
+    solutions are X_j = diag(1 / (l - z_j)), so moment block m is the diagonal
+    f_m(l) = -sum_j w_j t_j^m / (l - z_j), the trapezoid rule of
+    (1 / 2 pi i) contour-integral ((z - c) / rho)^m / (z - l) dz: it must be
+    ((l - c) / rho)^m inside the ellipse and ~0 outside.  A wrong sign of the
+    weights, a conjugated t_j or a dropped power fails it."""
+    st = ciss.Settings(block_size=8, moment_size=4)
+    lo, hi = -1.0, 1.0
+    z, t, w = ciss.contour(lo, hi, st)
+    c = 0.5 * (lo + hi)
+    rho = 0.5 * (hi - lo) + 2.0 * st.endpoint_guard_ev
+    lam = np.array([-0.8, -0.45, -0.1, 0.0, 0.2, 0.55, 0.85, -2.5, 1.9, 4.0], dtype=np.float64)
+    n = st.block_size
+    assert lam.size >= n
+    lam = lam[:n]
+    sols = [np.diag(1.0 / (lam - complex(zj))).astype(np.complex64) for zj in z]
+    M = ciss.moments_from_solutions(sols, t, w, st, n)
+    inside = np.abs(lam - c) < 0.9 * rho
+    # 32 nodes on a flat (0.1) ellipse: ~8 % ripple of f_0 inside, less on
+    # the higher moments, whose integrands vanish at the centre
+    tol = [0.1, 0.06, 0.06, 0.06]
+    for m in range(st.moment_size):
+        f = np.diag(M[:, m * n:(m + 1) * n])
+        want = np.where(inside, ((lam - c) / rho) ** m, 0.0)
+        assert np.all(np.abs(f[inside] - want[inside]) < tol[m]), (m, f, want)
+        assert np.all(np.abs(f[~inside]) < 1e-6), (m, f)
+        assert np.max(np.abs(M[:, m * n:(m + 1) * n] - np.diag(f))) == 0.0
+    # the same check fails for plausible slips: a conjugated t_j, a flipped
+    # weight sign
+    for bad_t, bad_w in ((np.conj(t), w), (t, -w)):
+        Mbad = ciss.moments_from_solutions(sols, bad_t, bad_w, st, n)
+        errs = [np.max(np.abs(np.diag(Mbad[:, m * n:(m + 1) * n])[inside] - ((lam[inside] - c) / rho) ** m))
+                for m in range(st.moment_size)]
+        assert max(errs) > 0.1
 
 
 def test_inertia_count_matches_eigenvalues(small2d, small2d_pattern):
