@@ -14,7 +14,6 @@ from .data import (
     LocalizedOperatorBlocks,
     LocalizedOperatorBundle,
     OrbitalBasisLayout,
-    StateRange,
 )
 from .errors import AOSpectrumError, InputError, SolverError
 
@@ -28,7 +27,6 @@ __all__ = [
     "LocalizedOperatorBundle",
     "OrbitalBasisLayout",
     "SolverError",
-    "StateRange",
     "load_bundle",
     "write_bundle",
 ]

@@ -1,19 +1,20 @@
 """``aospectrum band CONFIG [--resume]`` on the MI355X.
 
-Mirrors the band half of src/aospectrum/cli.py: TOML parsing
-(``read_band_config``, cli.py:123-177), the resumable work directory
-(``_prepare_work``, cli.py:275-296, with ``run.json`` identity cli.py:306-327),
-per-k checkpoints and the atomic publish of ``bands.csv``/``summary.json``/
-``band.png`` (cli.py:299-303, 368-436), and exit status 2 on
-``AOSpectrumError`` (cli.py:595-611).  The k loop hands ``[solver] k_batch``
-points at a time to the device.  Listing several ``[compute] devices`` stripes
-k points over one spawned process per GPU, as the reference does
-(cli.py:383-405); every process sees only its own device.
+The command-line contract is the reference's (src/aospectrum/cli.py): the
+TOML keys of the band configuration (cli.py:123-177), the resumable work
+directory next to the output with its ``run.json`` identity (cli.py:275-327),
+per-k checkpoints, the atomic publish of ``bands.csv``/``summary.json``/
+``band.png`` (cli.py:299-303, 368-436), device striping over ``[compute]
+devices`` (cli.py:383-405) and exit status 2 on an ``AOSpectrumError``
+(cli.py:595-611).  Here the configuration is read through a declarative key
+schema, the work directory is one object that prepares, resumes and
+publishes, and each device's worker hands ``[solver] k_batch`` points at a
+time to the GPU.
 
 Extra ``[solver]`` keys (defaults keep the reference's behaviour):
 ``precision``, ``refinement_steps``, ``refinement_passes``, ``k_batch``,
-``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``, ``autotune``,
-``resident_limit``.
+``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``,
+``autotune``, ``resident_limit``.
 """
 
 from __future__ import annotations
@@ -28,19 +29,22 @@ from concurrent.futures import ProcessPoolExecutor
 from dataclasses import asdict, dataclass
 from pathlib import Path
 from time import perf_counter
-from typing import Any, Sequence
+from typing import Any, Callable, Sequence
 
 try:
     import tomllib
 except ModuleNotFoundError:  # Python 3.10
     import tomli as tomllib
 
+from . import __version__, formats
 from .band import (
     BandCalculation,
+    BandResult,
     KPath,
     read_kpath,
     read_point,
     resolve_energy_reference,
+    scope_warnings,
     write_band_output,
     write_point,
 )
@@ -48,8 +52,6 @@ from .bundle import load_bundle
 from .ciss import CissSettings
 from .data import EnergyInterval, LocalizedOperatorBundle
 from .errors import AOSpectrumError, InputError, SolverError
-
-__version__ = "1.0.0rc2+mi355x"
 
 
 @dataclass(frozen=True, slots=True)
@@ -69,248 +71,289 @@ class BandConfig:
         return self.output.with_name(f".{self.output.name}.work")
 
 
-def _toml(path: str | Path) -> tuple[Path, dict[str, Any]]:
-    source = Path(path).expanduser().resolve()
-    try:
-        with source.open("rb") as handle:
-            return source, tomllib.load(handle)
-    except (OSError, tomllib.TOMLDecodeError) as exc:
-        raise InputError(f"cannot read TOML configuration {source}: {exc}") from exc
+# --------------------------------------------------------------- TOML keys
+class _Doc:
+    """A parsed TOML document with its tables checked on first use."""
+
+    def __init__(self, path: str | Path) -> None:
+        self.source = Path(path).expanduser().resolve()
+        try:
+            with self.source.open("rb") as handle:
+                self.values = tomllib.load(handle)
+        except (OSError, tomllib.TOMLDecodeError) as exc:
+            raise InputError(f"cannot read TOML configuration {self.source}: {exc}") from exc
+
+    def table(self, name: str) -> dict[str, Any]:
+        table = self.values.get(name, {})
+        if not isinstance(table, dict):
+            raise InputError(f"[{name}] must be a TOML table")
+        return table
+
+    def get(self, dotted: str, default: Any = None) -> Any:
+        table, key = dotted.split(".")
+        return self.table(table).get(key, default)
+
+    def path(self, dotted: str) -> Path:
+        value = self.get(dotted)
+        if not isinstance(value, str) or not value.strip():
+            raise InputError(f"{dotted} must be a path")
+        p = Path(value).expanduser()
+        return p.resolve() if p.is_absolute() else (self.source.parent / p).resolve()
+
+    def pair(self, dotted: str) -> tuple[float, float]:
+        value = self.get(dotted)
+        try:
+            if not isinstance(value, list) or len(value) != 2:
+                raise TypeError
+            return float(value[0]), float(value[1])
+        except (TypeError, ValueError) as exc:
+            raise InputError(f"{dotted} must contain two numbers") from exc
 
 
-def _section(values: dict[str, Any], name: str) -> dict[str, Any]:
-    section = values.get(name, {})
-    if not isinstance(section, dict):
-        raise InputError(f"[{name}] must be a TOML table")
-    return section
+def _boolean(value: Any) -> bool:
+    if not isinstance(value, bool):
+        raise TypeError("expected a boolean")
+    return value
 
 
-def _path(base: Path, value: Any, name: str) -> Path:
-    if not isinstance(value, str) or not value.strip():
-        raise InputError(f"{name} must be a path")
-    p = Path(value).expanduser()
-    return (base / p).resolve() if not p.is_absolute() else p.resolve()
+# CissSettings field <- [solver] key (type, default).  contour_batch_size may
+# also come from [resources], the reference's older location (cli.py:155-160).
+_SOLVER_KEYS: tuple[tuple[str, Callable[[Any], Any], Any], ...] = (
+    ("integration_points", int, 32),
+    ("contour_batch_size", int, None),
+    ("block_size", int, 32),
+    ("moment_size", int, 4),
+    ("precision", str, "fp32"),
+    ("refinement_steps", int, 0),
+    ("refinement_passes", int, 0),
+    ("k_batch", int, 1),
+    ("conjugate_pairs", _boolean, False),
+    ("refinement_policy", str, "fixed"),
+    ("residual_target_ev", float, 1.0e-4),
+    ("autotune", _boolean, False),
+    ("resident_limit", int, 0),
+)
 
 
-def _pair(value: Any, name: str) -> tuple[float, float]:
-    if not isinstance(value, list) or len(value) != 2:
-        raise InputError(f"{name} must contain two numbers")
-    try:
-        return float(value[0]), float(value[1])
-    except (TypeError, ValueError) as exc:
-        raise InputError(f"{name} must contain two numbers") from exc
+def _solver_settings(doc: _Doc) -> CissSettings:
+    solver = doc.table("solver")
+    fallback_batch = doc.table("resources").get("contour_batch_size", 1)
+    values = {}
+    for name, kind, default in _SOLVER_KEYS:
+        raw = solver.get(name, fallback_batch if name == "contour_batch_size" else default)
+        values[name] = kind(raw)
+    return CissSettings(**values)
+
+
+def _devices(doc: _Doc) -> tuple[int, ...]:
+    devices = doc.get("compute.devices", [0])
+    valid = (
+        isinstance(devices, list)
+        and bool(devices)
+        and all(isinstance(d, int) and not isinstance(d, bool) and d >= 0 for d in devices)
+        and len(set(devices)) == len(devices)
+    )
+    if not valid:
+        raise InputError("compute.devices must contain unique GPU indices")
+    return tuple(devices)
 
 
 def read_band_config(path: str | Path) -> BandConfig:
-    """(cli.py:123-177) plus the device-execution keys of ``[solver]``."""
-    source, values = _toml(path)
-    base = source.parent
-    inp, band = _section(values, "input"), _section(values, "band")
-    compute, solver = _section(values, "compute"), _section(values, "solver")
-    resources, output = _section(values, "resources"), _section(values, "output")
-    devices = compute.get("devices", [0])
-    if (
-        not isinstance(devices, list)
-        or not devices
-        or any(isinstance(v, bool) or not isinstance(v, int) or v < 0 for v in devices)
-        or len(set(devices)) != len(devices)
-    ):
-        raise InputError("compute.devices must contain unique GPU indices")
-    interval = EnergyInterval(*_pair(band.get("energy_interval_ev"), "band.energy_interval_ev"))
-    mode = str(band.get("energy_reference", "input_zero"))
-    ref_value = band.get("energy_reference_ev")
+    """The band configuration (cli.py:123-177) with the device-execution keys."""
+    doc = _Doc(path)
+    devices = _devices(doc)
+    interval = EnergyInterval(*doc.pair("band.energy_interval_ev"))
     try:
+        ref_value = doc.get("band.energy_reference_ev")
         ref_value = None if ref_value is None else float(ref_value)
-        conj = solver.get("conjugate_pairs", False)
-        tune = solver.get("autotune", False)
-        if not isinstance(conj, bool) or not isinstance(tune, bool):
-            raise TypeError("conjugate_pairs and autotune must be booleans")
-        settings = CissSettings(
-            integration_points=int(solver.get("integration_points", 32)),
-            contour_batch_size=int(solver.get("contour_batch_size", resources.get("contour_batch_size", 1))),
-            block_size=int(solver.get("block_size", 32)),
-            moment_size=int(solver.get("moment_size", 4)),
-            precision=str(solver.get("precision", "fp32")),
-            refinement_steps=int(solver.get("refinement_steps", 0)),
-            refinement_passes=int(solver.get("refinement_passes", 0)),
-            k_batch=int(solver.get("k_batch", 1)),
-            conjugate_pairs=conj,
-            refinement_policy=str(solver.get("refinement_policy", "fixed")),
-            residual_target_ev=float(solver.get("residual_target_ev", 1.0e-4)),
-            autotune=tune,
-            resident_limit=int(solver.get("resident_limit", 0)),
-        )
+        settings = _solver_settings(doc)
     except (TypeError, ValueError) as exc:
         raise InputError("invalid Band solver setting") from exc
     return BandConfig(
-        source,
-        _path(base, inp.get("bundle"), "input.bundle"),
-        read_kpath(_path(base, band.get("kpath"), "band.kpath")),
-        interval,
-        mode,
-        ref_value,
-        tuple(devices),
-        settings,
-        _path(base, output.get("directory"), "output.directory"),
+        source=doc.source,
+        bundle=doc.path("input.bundle"),
+        kpath=read_kpath(doc.path("band.kpath")),
+        interval=interval,
+        reference_mode=str(doc.get("band.energy_reference", "input_zero")),
+        reference_value_ev=ref_value,
+        devices=devices,
+        settings=settings,
+        output=doc.path("output.directory"),
     )
 
 
-def _device_token(index: int, inherited: str | None) -> str:
-    """(cli.py:259-267) for HIP_VISIBLE_DEVICES."""
-    if inherited:
-        visible = [v.strip() for v in inherited.split(",") if v.strip()]
-        if index >= len(visible):
-            raise InputError(f"GPU index {index} exceeds HIP_VISIBLE_DEVICES={inherited}")
-        return visible[index]
-    return str(index)
+# ------------------------------------------------------------ work directory
+class WorkDirectory:
+    """The hidden ``.<output>.work`` directory: created with the run's
+    identity, reopened only for the same identity on --resume, and replaced
+    by the published output when every k point is done."""
 
+    def __init__(self, output: Path) -> None:
+        self.output = output
+        self.root = output.with_name(f".{output.name}.work")
+        self.staging = output.with_name(f".{output.name}.publish")
 
-def _require_output(path: Path) -> None:
-    if path.exists() and (not path.is_dir() or any(path.iterdir())):
-        raise InputError(f"output directory is not empty: {path}")
+    def check_output_free(self) -> None:
+        out = self.output
+        if out.exists() and (not out.is_dir() or any(out.iterdir())):
+            raise InputError(f"output directory is not empty: {out}")
+
+    def open(self, identity: dict[str, Any], *, resume: bool) -> None:
+        run = self.root / formats.RUN_DOCUMENT
+        if not resume:
+            if self.root.exists():
+                raise InputError(f"work directory already exists: {self.root}; use --resume")
+            self.root.mkdir(parents=True)
+            run.write_text(formats.json_text(identity), encoding="utf-8")
+            return
+        try:
+            recorded = json.loads(run.read_text(encoding="utf-8"))
+        except (OSError, json.JSONDecodeError) as exc:
+            raise InputError(f"cannot resume work directory {self.root}") from exc
+        if recorded != identity:
+            raise InputError("resume work belongs to another calculation")
+
+    def checkpoint(self, index: int) -> Path:
+        return self.root / formats.CHECKPOINT_PATTERN.format(index=index)
+
+    def publish(self, write: Callable[[Path], Any]) -> None:
+        if self.staging.exists():
+            shutil.rmtree(self.staging)
+        write(self.staging)
+        if self.output.exists():
+            self.output.rmdir()  # empty, checked before the run
+        os.replace(self.staging, self.output)
+        shutil.rmtree(self.root)
 
 
 def _prepare_work(path: Path, spec: dict[str, Any], *, resume: bool) -> None:
-    """(cli.py:275-296)"""
-    run = path / "run.json"
-    if resume:
-        try:
-            observed = json.loads(run.read_text(encoding="utf-8"))
-        except (OSError, json.JSONDecodeError) as exc:
-            raise InputError(f"cannot resume work directory {path}") from exc
-        if observed != spec:
-            raise InputError("resume work belongs to another calculation")
-        return
-    if path.exists():
-        raise InputError(f"work directory already exists: {path}; use --resume")
-    path.mkdir(parents=True)
-    run.write_text(json.dumps(spec, indent=2, sort_keys=True) + "\n", encoding="utf-8")
-
-
-def _publish_directory(output: Path, publish: Path, work: Path) -> None:
-    if output.exists():
-        output.rmdir()
-    os.replace(publish, output)
-    shutil.rmtree(work)
+    """Create or reopen the work directory ``path`` (the reference's
+    _prepare_work contract, cli.py:275-296)."""
+    work = WorkDirectory(path.with_name(path.name.removeprefix(".").removesuffix(".work")))
+    work.root = path
+    work.open(spec, resume=resume)
 
 
 def _band_specification(config: BandConfig, bundle: LocalizedOperatorBundle) -> dict[str, Any]:
-    """run.json identity (cli.py:306-327)."""
-    return {
-        "kind": "band",
-        "bundle": str(config.bundle),
-        "basis_identity": bundle.basis.basis_identity,
-        "n_orbitals": bundle.n_orbitals,
-        "interval_ev": [config.interval.lower_ev, config.interval.upper_ev],
-        "reference": [config.reference_mode, config.reference_value_ev],
-        "kpoints": config.kpath.fractional_points.tolist(),
-        "nodes": config.kpath.node_indices.tolist(),
-        "labels": list(config.kpath.node_labels),
-        "solver": asdict(config.settings),
-    }
+    """The identity a resumed run must match (run.json, cli.py:306-327)."""
+    path = config.kpath
+    return dict(
+        kind="band",
+        bundle=str(config.bundle),
+        basis_identity=bundle.basis.basis_identity,
+        n_orbitals=bundle.n_orbitals,
+        interval_ev=[config.interval.lower_ev, config.interval.upper_ev],
+        reference=[config.reference_mode, config.reference_value_ev],
+        kpoints=path.fractional_points.tolist(),
+        nodes=path.node_indices.tolist(),
+        labels=list(path.node_labels),
+        solver=asdict(config.settings),
+    )
+
+
+# --------------------------------------------------------------- execution
+def _visible_token(index: int, inherited: str | None) -> str:
+    """The HIP_VISIBLE_DEVICES entry of configured device ``index``: a
+    position in an inherited list, else the ordinal itself (cli.py:259-267)."""
+    if not inherited:
+        return str(index)
+    visible = [v.strip() for v in inherited.split(",") if v.strip()]
+    if index >= len(visible):
+        raise InputError(f"GPU index {index} exceeds HIP_VISIBLE_DEVICES={inherited}")
+    return visible[index]
 
 
 def _band_worker(config: BandConfig, reference: float, rank: int, world: int, token: str | None) -> int:
-    """Solve this rank's stripe of k points (cli.py:330-365), k_batch at a time."""
+    """Solve the k points i = rank (mod world) that have no checkpoint yet,
+    ``k_batch`` at a time on this worker's device; returns the points this
+    stripe holds afterwards."""
     if token is not None:
         os.environ["HIP_VISIBLE_DEVICES"] = token
-    bundle = load_bundle(config.bundle)
-    calc = BandCalculation(bundle, config.kpath, config.interval, reference, config.settings)
-    done = 0
+    work = WorkDirectory(config.output)
+    stripe = range(rank, config.kpath.n_points, world)
+    todo = [i for i in stripe
+            if read_point(work.checkpoint(i), index=i, k_fractional=config.kpath.fractional_points[i]) is None]
+    calc = BandCalculation(load_bundle(config.bundle), config.kpath, config.interval, reference, config.settings)
     try:
-        pending = []
-        for index in range(rank, config.kpath.n_points, world):
-            path = config.work / f"k-{index:06d}.npz"
-            if read_point(path, index=index, k_fractional=config.kpath.fractional_points[index]) is None:
-                pending.append(index)
-            else:
-                done += 1
         step = max(1, config.settings.k_batch)
-        for start in range(0, len(pending), step):
-            chunk = pending[start:start + step]
-            print(
-                f"[gpu {rank}] k points {chunk[0] + 1}..{chunk[-1] + 1}/{config.kpath.n_points}",
-                flush=True,
-            )
-            for point in calc.solve_points(chunk):
-                write_point(config.work / f"k-{point.index:06d}.npz", point)
-                done += 1
+        for batch in (todo[j:j + step] for j in range(0, len(todo), step)):
+            print(f"[gpu {rank}] k points {batch[0] + 1}..{batch[-1] + 1}/{config.kpath.n_points}", flush=True)
+            for point in calc.solve_points(batch):
+                write_point(work.checkpoint(point.index), point)
     finally:
         calc.close()
-    return done
+    return len(stripe)
 
 
-def run_band(path: str | Path, *, resume: bool = False) -> Path:
-    """(cli.py:368-436)"""
-    config = read_band_config(path)
-    _require_output(config.output)
-    bundle = load_bundle(config.bundle)
-    reference = resolve_energy_reference(bundle, config.reference_mode, config.reference_value_ev)
-    _prepare_work(config.work, _band_specification(config, bundle), resume=resume)
-    started = perf_counter()
+def _run_workers(config: BandConfig, reference: float) -> None:
     inherited = os.environ.get("HIP_VISIBLE_DEVICES")
     if len(config.devices) == 1:
-        token = _device_token(config.devices[0], inherited) if config.devices[0] != 0 or inherited else None
+        only = config.devices[0]
+        token = _visible_token(only, inherited) if (only != 0 or inherited) else None
         _band_worker(config, reference, 0, 1, token)
-    else:
-        tokens = [_device_token(i, inherited) for i in config.devices]
-        ctx = multiprocessing.get_context("spawn")
-        with ProcessPoolExecutor(max_workers=len(tokens), mp_context=ctx) as pool:
-            futures = [
-                pool.submit(_band_worker, config, reference, rank, len(tokens), token)
-                for rank, token in enumerate(tokens)
-            ]
-            for f in futures:
-                f.result()
+        return
+    tokens = [_visible_token(i, inherited) for i in config.devices]
+    with ProcessPoolExecutor(max_workers=len(tokens), mp_context=multiprocessing.get_context("spawn")) as pool:
+        jobs = [pool.submit(_band_worker, config, reference, r, len(tokens), t) for r, t in enumerate(tokens)]
+        for job in jobs:
+            job.result()
+
+
+def _collect(config: BandConfig, work: WorkDirectory) -> list:
     points = []
-    for index, kpoint in enumerate(config.kpath.fractional_points):
-        point = read_point(config.work / f"k-{index:06d}.npz", index=index, k_fractional=kpoint)
+    for index, k in enumerate(config.kpath.fractional_points):
+        point = read_point(work.checkpoint(index), index=index, k_fractional=k)
         if point is None:
             raise SolverError(f"Band point {index} is missing")
         points.append(point)
-    warnings = []
-    if bundle.basis.spinor_width != 1:
-        warnings.append("spinor input is outside the current non-SOC Band scope; the result may be inaccurate")
-    if not all(bundle.structure.periodic):
-        warnings.append("non-periodic axes are outside the current Bloch scope; the result may be inaccurate")
-    from .band import BandResult
+    return points
 
-    solve_interval = config.interval.expanded(0.02)
+
+def run_band(path: str | Path, *, resume: bool = False) -> Path:
+    """Solve every k point of the configured path, resumably, and publish."""
+    config = read_band_config(path)
+    work = WorkDirectory(config.output)
+    work.check_output_free()
+    bundle = load_bundle(config.bundle)
+    reference = resolve_energy_reference(bundle, config.reference_mode, config.reference_value_ev)
+    work.open(_band_specification(config, bundle), resume=resume)
+    started = perf_counter()
+    _run_workers(config, reference)
     result = BandResult(
         kpath=config.kpath,
         display_interval_ev=config.interval,
-        solve_interval_ev=solve_interval,
+        solve_interval_ev=config.interval.expanded(0.02),
         reference_energy_ev=reference,
         cell_angstrom=bundle.structure.cell_angstrom,
-        points=tuple(points),
+        points=tuple(_collect(config, work)),
         elapsed_seconds=perf_counter() - started,
-        warnings=tuple(warnings),
+        warnings=scope_warnings(bundle),
     )
-    publish = config.output.with_name(f".{config.output.name}.publish")
-    if publish.exists():
-        shutil.rmtree(publish)
-    write_band_output(result, publish)
-    _publish_directory(config.output, publish, config.work)
+    work.publish(lambda staging: write_band_output(result, staging))
     print(config.output)
     return config.output
+
+
+# --------------------------------------------------------------------- CLI
+_COMMANDS: dict[str, Callable[[argparse.Namespace], Any]] = {
+    "band": lambda args: run_band(args.config, resume=args.resume),
+}
 
 
 def _parser() -> argparse.ArgumentParser:
     parser = argparse.ArgumentParser(prog="aospectrum", description="Band path for localized AO Hamiltonians (MI355X).")
     parser.add_argument("--version", action="version", version=__version__)
-    commands = parser.add_subparsers(dest="command", required=True)
-    band = commands.add_parser("band", help="calculate an energy-window band structure")
+    sub = parser.add_subparsers(dest="command", required=True)
+    band = sub.add_parser("band", help="calculate an energy-window band structure")
     band.add_argument("config", type=Path)
     band.add_argument("--resume", action="store_true")
     return parser
 
 
 def main(argv: Sequence[str] | None = None) -> int:
-    """(cli.py:595-611)"""
     args = _parser().parse_args(argv)
     try:
-        if args.command == "band":
-            run_band(args.config, resume=args.resume)
+        _COMMANDS[args.command](args)
     except AOSpectrumError as exc:
         print(f"aospectrum: {exc}", file=sys.stderr)
         return 2

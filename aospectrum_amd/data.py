@@ -275,22 +275,3 @@ class EnergyInterval:
 
     def expanded(self, guard_ev: float) -> "EnergyInterval":
         return EnergyInterval(self.lower_ev - float(guard_ev), self.upper_ev + float(guard_ev))
-
-
-@dataclass(frozen=True, slots=True)
-class StateRange:
-    """1-based inclusive state range (data.py:337-352)."""
-
-    first: int
-    last: int
-
-    def __post_init__(self) -> None:
-        first, last = int(self.first), int(self.last)
-        if first < 1 or last < first:
-            raise InputError("state range requires 1 <= first <= last")
-        object.__setattr__(self, "first", first)
-        object.__setattr__(self, "last", last)
-
-    @property
-    def count(self) -> int:
-        return self.last - self.first + 1
