@@ -281,6 +281,17 @@ class ResidualRequest(C.Structure):
     ]
 
 
+class AssembleNoallocRequest(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32),
+        ("struct_size", C.c_uint32),
+        ("n_kpoints", C.c_int64),
+        ("k_fractional", _F64P),
+        ("h_out", C.c_void_p),
+        ("s_out", C.c_void_p),
+    ]
+
+
 _STRUCTS = (
     BundleRequest,
     BundleResult,
@@ -294,6 +305,7 @@ _STRUCTS = (
     AutotuneRequest,
     AutotuneResult,
     ResidualRequest,
+    AssembleNoallocRequest,
 )
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "csrc" / "libaosx_hip.so"
@@ -337,6 +349,7 @@ def load() -> C.CDLL:
         "aosx_bundle_trim_v1": [_P, C.POINTER(C.c_uint64)],
         "aosx_bundle_autotune_v1": [_P, C.POINTER(AutotuneRequest), C.POINTER(AutotuneResult)],
         "aosx_eig_residual_v1": [_P, C.POINTER(ResidualRequest), C.POINTER(StatusResult)],
+        "aosx_assemble_noalloc_v1": [_P, C.POINTER(AssembleNoallocRequest), C.POINTER(StatusResult)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -358,6 +371,7 @@ EXPORTED_SYMBOLS = (
     "aosx_bundle_trim_v1",
     "aosx_bundle_autotune_v1",
     "aosx_eig_residual_v1",
+    "aosx_assemble_noalloc_v1",
 )
 
 

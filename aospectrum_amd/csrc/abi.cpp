@@ -242,6 +242,52 @@ int aosx_assemble_v1(aosx_bundle *bundle, const aosx_assemble_request_v1 *q, aos
     });
 }
 
+int aosx_assemble_noalloc_v1(aosx_bundle *bundle, const aosx_assemble_noalloc_request_v1 *q, aosx_status_v1 *res) {
+    if (!q || !res) return 1;
+    begin(res);
+    if (!abi_ok(q)) return fail(res, AOSX_ABI_MISMATCH, 0, "request ABI mismatch");
+    if (!bundle || !q->k_fractional || !q->h_out || !q->s_out)
+        return fail(res, AOSX_NULL_POINTER, 0, "required input or output pointer is null");
+    Bundle &b = *bundle->impl;
+    if (b.device < 0) return fail(res, AOSX_NO_DEVICE, 0, "bundle was created without a device");
+    if (q->n_kpoints < 1 || q->n_kpoints > 64)
+        return fail(res, AOSX_BATCH_DIMENSIONS, 0, "n_kpoints must be in [1, 64] for the allocation-free entry point");
+    if ((uint64_t)q->n_kpoints * (uint64_t)b.n_groups * 8u > AOSX_NOALLOC_SCRATCH_BYTES)
+        return fail(res, AOSX_BATCH_DIMENSIONS, 0,
+                    "n_kpoints x n_phase_shifts x 8 bytes exceeds the %u-byte scratch array",
+                    (unsigned)AOSX_NOALLOC_SCRATCH_BYTES);
+    for (int64_t i = 0; i < 3 * q->n_kpoints; ++i)
+        if (!std::isfinite(q->k_fractional[i])) return fail(res, AOSX_NONFINITE_VALUES, 0, "k point is not finite");
+    return guarded(res, [&]() -> int {
+        double t0 = now();
+        AOSX_HIP(hipSetDevice(b.device));
+        // the outputs must be device allocations on the bundle's device large
+        // enough for n_kpoints x nnz values
+        const size_t need = (size_t)q->n_kpoints * b.nnz * sizeof(aosx_complex64);
+        for (void *out : {(void *)q->h_out, (void *)q->s_out}) {
+            hipPointerAttribute_t attr;
+            if (hipPointerGetAttributes(&attr, out) != hipSuccess || attr.type != hipMemoryTypeDevice ||
+                attr.device != b.device) {
+                (void)hipGetLastError();
+                return fail(res, AOSX_NULL_POINTER, 0, "h_out and s_out must be device memory on device %d",
+                            b.device);
+            }
+            hipDeviceptr_t base = nullptr;
+            size_t size = 0;
+            if (hipMemGetAddressRange(&base, &size, out) != hipSuccess ||
+                (char *)out + need > (char *)base + size) {
+                (void)hipGetLastError();
+                return fail(res, AOSX_BATCH_DIMENSIONS, 0,
+                            "output buffer holds fewer than n_kpoints x nnz complex64 values");
+            }
+        }
+        b.assemble_noalloc(q->k_fractional, (int)q->n_kpoints, reinterpret_cast<float2 *>(q->h_out),
+                           reinterpret_cast<float2 *>(q->s_out));
+        res->seconds = now() - t0;
+        return 0;
+    });
+}
+
 int aosx_shift_solve_v1(aosx_bundle *bundle, const aosx_shift_request_v1 *q, aosx_shift_result_v1 *res) {
     if (!q || !res) return 1;
     begin(res);
@@ -502,7 +548,8 @@ int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity) {
                            (int64_t)sizeof(aosx_shift_request_v1),  (int64_t)sizeof(aosx_shift_result_v1),
                            (int64_t)sizeof(aosx_band_request_v1),   (int64_t)sizeof(aosx_band_result_v1),
                            (int64_t)sizeof(aosx_kernel_stats_v1),   (int64_t)sizeof(aosx_autotune_request_v1),
-                           (int64_t)sizeof(aosx_autotune_result_v1), (int64_t)sizeof(aosx_residual_request_v1)};
+                           (int64_t)sizeof(aosx_autotune_result_v1), (int64_t)sizeof(aosx_residual_request_v1),
+                           (int64_t)sizeof(aosx_assemble_noalloc_request_v1)};
     const int count = (int)(sizeof(all) / sizeof(all[0]));
     for (int i = 0; i < count && i < capacity; ++i) sizes[i] = all[i];
     return count;

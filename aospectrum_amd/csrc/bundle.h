@@ -194,6 +194,10 @@ struct Bundle {
                 const int64_t *atom_offsets, int64_t atoms);
     // pipeline stages
     void assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *d_s);
+    // Allocation-free assembly into caller-owned device buffers: the phases
+    // go to this device's static scratch array (no workspace, no copies).
+    void *noalloc_scratch = nullptr;  // device address of that array
+    void assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, float2 *d_s);
     void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
                 float *tiles, float *dinv, int *d_replaced);
     // precision selects the MFMA form of the solve products (BF16 split for

@@ -11,18 +11,35 @@
 
 namespace aosx {
 
+// exp(2 pi i R . k) in FP64, rounded to complex64
+__device__ __forceinline__ float2 bloch_phase(const int32_t *__restrict__ R, const double *k) {
+    const double arg = 2.0 * 3.14159265358979323846 * (R[0] * k[0] + R[1] * k[1] + R[2] * k[2]);
+    double sn, cs;
+    sincos(arg, &sn, &cs);
+    return make_float2((float)cs, (float)sn);
+}
+
 // phase[k][g] = exp(2 pi i R_g . k)
 __global__ void k_phases(float2 *__restrict__ phase, const int32_t *__restrict__ shifts, int n_groups,
                          const double *__restrict__ kpts, int nk) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_groups * nk) return;
     const int k = e / n_groups, g = e % n_groups;
-    const double arg = 2.0 * 3.14159265358979323846 *
-                       (shifts[3 * g] * kpts[3 * k] + shifts[3 * g + 1] * kpts[3 * k + 1] +
-                        shifts[3 * g + 2] * kpts[3 * k + 2]);
-    double sn, cs;
-    sincos(arg, &sn, &cs);
-    phase[e] = make_float2((float)cs, (float)sn);
+    phase[e] = bloch_phase(shifts + 3 * g, kpts + 3 * k);
+}
+
+// The same phases for at most KBATCH_MAX k points passed by value (no device
+// copy of the k points).
+constexpr int KBATCH_MAX = 64;
+struct KBatch {
+    double k[KBATCH_MAX][3];
+};
+__global__ void k_phases_byval(float2 *__restrict__ phase, const int32_t *__restrict__ shifts, int n_groups,
+                               const KBatch kpts, int nk) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_groups * nk) return;
+    const int k = e / n_groups, g = e % n_groups;
+    phase[e] = bloch_phase(shifts + 3 * g, kpts.k[k]);
 }
 
 __device__ __forceinline__ float2 bloch_sum(const float2 *__restrict__ vals, const int32_t *__restrict__ source,

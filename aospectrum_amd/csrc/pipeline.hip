@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "bundle.h"
 #include "kernels_asm.hip"
@@ -176,6 +177,30 @@ void Bundle::run_sequence(const std::vector<int64_t> &key_in, F &&body) {
     if (!it->second.ranges.empty()) flush_ranges(it->second.ranges, false);
 }
 
+// Scratch of aosx_assemble_noalloc_v1: the Bloch phases of one call,
+// n_kpoints x n_phase_shifts complex64.  Each device holds its own copy until
+// the process ends; calls that overlap on one device share it and race.
+static __device__ __align__(256) unsigned char scratch[AOSX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mutex;
+void *g_scratch_address[AOSX_MAX_DEVICES] = {};
+}  // namespace
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device: in host code the array's name is only a stand-in.
+static void *scratch_address(int device) {
+    if (device < 0 || device >= AOSX_MAX_DEVICES) throw Error(AOSX_NO_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void *ptr = nullptr;
+        AOSX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Error(AOSX_HIP_ERROR, "scratch symbol has no device address");
+        g_scratch_address[device] = ptr;
+    }
+    return g_scratch_address[device];
+}
+
 Bundle::~Bundle() {
     if (device >= 0) {
         (void)hipSetDevice(device);
@@ -293,6 +318,7 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     if (const char *env = std::getenv("AOSX_GRAPHS")) use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("AOSX_QUAD_PIPELINE")) quad_pipelined = std::atoi(env) != 0;
     peak_bytes = resident_bytes;
+    noalloc_scratch = scratch_address(device);
 }
 
 // Step groups of `group` consecutive elimination steps and the trailing pair
@@ -552,6 +578,25 @@ void Bundle::assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *
         const double src = (double)pat.source.size();
         kstats.assemble_bytes += nk * (2.0 * 2.0 * src * (8.0 + 4.0 + 4.0) + nnz * (2 * 8.0 + 4 * 4.0 + 4.0));
     }
+}
+
+// ----------------------------------------------------- allocation-free assembly
+void Bundle::assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, float2 *d_s) {
+    if (nk < 1 || nk > KBATCH_MAX) throw Error(AOSX_BATCH_DIMENSIONS, "n_kpoints must be in [1, 64]");
+    if ((uint64_t)nk * (uint64_t)n_groups * sizeof(float2) > AOSX_NOALLOC_SCRATCH_BYTES)
+        throw Error(AOSX_BATCH_DIMENSIONS, "n_kpoints x n_phase_shifts exceeds the scratch array");
+    if (!noalloc_scratch) throw Error(AOSX_NO_DEVICE, "bundle has no scratch address");
+    KBatch kb{};
+    for (int k = 0; k < nk; ++k)
+        for (int d = 0; d < 3; ++d) kb.k[k][d] = kpts_host[3 * k + d];
+    float2 *phase = static_cast<float2 *>(noalloc_scratch);
+    const int ng = (int)n_groups;
+    k_phases_byval<<<blocks_for((int64_t)ng * nk, 256, 1 << 20), 256, 0, stream>>>(phase, d_shifts, ng, kb, nk);
+    AOSX_LAUNCHED();
+    k_assemble<<<dim3(blocks_for(nnz, 256, 4096), (unsigned)nk), 256, 0, stream>>>(
+        d_h, d_s, d_hval, d_sval, d_source, d_group, d_starts, d_mirror, phase, ng, nnz);
+    AOSX_LAUNCHED();
+    AOSX_HIP(hipStreamSynchronize(stream));
 }
 
 // ------------------------------------------------------------- factorization

@@ -183,6 +183,31 @@ typedef struct {
 int aosx_assemble_v1(aosx_bundle *bundle, const aosx_assemble_request_v1 *request,
                      aosx_status_v1 *result);
 
+/* ------------------------------------------------- allocation-free assembly
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the same H(k), S(k) as aosx_assemble_v1, written into
+ * caller-owned device buffers, with no device allocation or free.  The k
+ * points travel as kernel arguments and the Bloch phases are staged in a
+ * static device array of AOSX_NOALLOC_SCRATCH_BYTES per device, so
+ * n_kpoints <= 64 and n_kpoints * n_phase_shifts * 8 <= that size; larger
+ * requests fail with AOSX_BATCH_DIMENSIONS.  Calls that overlap on one device
+ * share the array and race; the call returns after its kernels finished. */
+#define AOSX_NOALLOC_SCRATCH_BYTES (4u << 20)
+#define AOSX_MAX_DEVICES 64
+
+typedef struct {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    int64_t n_kpoints;             /* 1..64                                        */
+    const double *k_fractional;    /* [n_kpoints][3] host                          */
+    aosx_complex64 *h_out;         /* [n_kpoints][nnz] device memory of the bundle's */
+    aosx_complex64 *s_out;         /* device, CSR order                             */
+} aosx_assemble_noalloc_request_v1;
+
+int aosx_assemble_noalloc_v1(aosx_bundle *bundle, const aosx_assemble_noalloc_request_v1 *request,
+                             aosx_status_v1 *result);
+
 /* ------------------------------------------------------------- shift solve */
 typedef struct {
     uint32_t abi_version;
@@ -381,8 +406,8 @@ int aosx_bundle_autotune_v1(aosx_bundle *bundle, const aosx_autotune_request_v1 
 /* sizeof of every request/result struct, so bindings can verify their
  * layouts, in this order: bundle request, bundle result, status, assemble
  * request, shift request, shift result, band request, band result, kernel
- * stats, autotune request, autotune result, residual request; returns the
- * number of sizes written. */
+ * stats, autotune request, autotune result, residual request, allocation-
+ * free assemble request; returns the number of sizes written. */
 int aosx_struct_sizes_v1(int64_t *sizes, int32_t capacity);
 
 #ifdef __cplusplus
