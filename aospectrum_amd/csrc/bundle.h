@@ -122,6 +122,7 @@ struct Bundle {
     // the same trailing targets as 2 x 2 blocks (i0, i1, j0, j1; -1 absent)
     std::vector<int64_t> grp_quad_ptr;
     int32_t *d_quads = nullptr;
+
     // Step masks, bit t - g0 for step t of the group: per pair (trailing and
     // Crout lists alike) the steps with both operands L(i,t), U(t,j); per quad
     // the steps with L(i0,t), L(i1,t), U(t,j0), U(t,j1) (four masks).
@@ -149,9 +150,9 @@ struct Bundle {
     // the first call with a given key (sizes, precision, buffer addresses)
     // and replayed afterwards; AOSX_GRAPHS=0 runs them eagerly.
     bool use_graphs = true;
-    // trailing quad updates pipelined by K halves (AOSX_QUAD_PIPELINE=0: the
-    // step-at-a-time kernel)
-    bool quad_pipelined = true;
+    // trailing quad update form (AOSX_QUAD_PIPELINE): 0 the step-at-a-time
+    // kernel, 1 K quarters double-buffered through LDS
+    int quad_pipeline = 1;
     std::map<std::vector<int64_t>, GraphEntry> graphs;
     std::vector<TimedRange> *ranges = nullptr;  // collecting ranges of the running sequence
     bool capturing = false;

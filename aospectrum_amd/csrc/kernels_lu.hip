@@ -883,24 +883,40 @@ __device__ __forceinline__ void qq_issue(uint16_t *buf, const uint16_t *__restri
 __device__ __forceinline__ bf16x8 lds8_and(const uint16_t *p, uint32_t keep) {
     return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, lds8(p)) & (u32x4)(keep));
 }
-__device__ __forceinline__ void qq_mma(const uint16_t *buf, int ua, int ub, floatx16 (&cre)[4], floatx16 (&cim)[4],
-                                       int lane, bool three, uint32_t keep) {
+// The 16 operand blocks of one quarter buffer, read into registers before
+// any product (A from tile ua, masked by `keep`; B from tile ub).
+struct QuarterOps {
+    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2];
+};
+__device__ __forceinline__ QuarterOps qq_read(const uint16_t *buf, int ua, int ub, int lane, bool three,
+                                              uint32_t keep) {
     auto blk = [&](int u, int pl, int a) { return buf + (u * 8 + pl * 2 + a) * SPLIT_BLOCK + lane * 8; };
-    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2], npih[2], npil[2];
+    QuarterOps o;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        prh[h] = lds8_and(blk(ua, 0, h), keep);
-        pih[h] = lds8_and(blk(ua, 2, h), keep);
-        urh[h] = lds8(blk(ub, 0, h));
-        uih[h] = lds8(blk(ub, 2, h));
+        o.prh[h] = lds8_and(blk(ua, 0, h), keep);
+        o.pih[h] = lds8_and(blk(ua, 2, h), keep);
+        o.urh[h] = lds8(blk(ub, 0, h));
+        o.uih[h] = lds8(blk(ub, 2, h));
         if (three) {
-            prl[h] = lds8_and(blk(ua, 1, h), keep);
-            pil[h] = lds8_and(blk(ua, 3, h), keep);
-            url[h] = lds8(blk(ub, 1, h));
-            uil[h] = lds8(blk(ub, 3, h));
+            o.prl[h] = lds8_and(blk(ua, 1, h), keep);
+            o.pil[h] = lds8_and(blk(ua, 3, h), keep);
+            o.url[h] = lds8(blk(ub, 1, h));
+            o.uil[h] = lds8(blk(ub, 3, h));
         }
-        npih[h] = neg8(pih[h]);
-        npil[h] = three ? neg8(pil[h]) : npih[h];
+    }
+    return o;
+}
+// The products of one quarter.  Each of the 8 accumulators takes its six
+// products in the order of mma_split_quad_chunk; the accumulators take turns,
+// so consecutive MFMAs never depend on each other.
+__device__ __forceinline__ void qq_products(const QuarterOps &o, floatx16 (&cre)[4], floatx16 (&cim)[4],
+                                            bool three) {
+    bf16x8 npih[2], npil[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        npih[h] = neg8(o.pih[h]);
+        npil[h] = three ? neg8(o.pil[h]) : npih[h];
     }
     const int np = three ? 6 : 2;
 #pragma unroll
@@ -909,31 +925,29 @@ __device__ __forceinline__ void qq_mma(const uint16_t *buf, int ua, int ub, floa
         for (int t = 0; t < 4; ++t) {
             const int a = t >> 1, b = t & 1;
             // re += Pr Ur - Pi Ui (hi hi, then hi lo, lo hi), im += Pr Ui + Pi Ur
-            const bf16x8 xr[6] = {prh[a], npih[a], prh[a], prl[a], npih[a], npil[a]};
-            const bf16x8 yr[6] = {urh[b], uih[b], url[b], urh[b], uil[b], uih[b]};
-            const bf16x8 xi[6] = {prh[a], pih[a], prh[a], prl[a], pih[a], pil[a]};
-            const bf16x8 yi[6] = {uih[b], urh[b], uil[b], uih[b], url[b], urh[b]};
+            const bf16x8 xr[6] = {o.prh[a], npih[a], o.prh[a], o.prl[a], npih[a], npil[a]};
+            const bf16x8 yr[6] = {o.urh[b], o.uih[b], o.url[b], o.urh[b], o.uil[b], o.uih[b]};
+            const bf16x8 xi[6] = {o.prh[a], o.pih[a], o.prh[a], o.prl[a], o.pih[a], o.pil[a]};
+            const bf16x8 yi[6] = {o.uih[b], o.urh[b], o.uil[b], o.uih[b], o.url[b], o.urh[b]};
             cre[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xr[p], yr[p], cre[t], 0, 0, 0);
             cim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xi[p], yi[p], cim[t], 0, 0, 0);
         }
 }
 
+// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0] | expcnt << 4 | lgkmcnt << 8
+// | vmcnt[5:4] << 14) that wait for one counter only
+constexpr int WAIT_VM8 = 0x0F78, WAIT_VM0 = 0x0F70, WAIT_LGKM0 = 0xC07F;
+
+// One quad (matrix m, quad q) of the trailing update, K quarters double-
+// buffered through lds0/lds1 (see k_schur_quad_pp).  Called by every wave of
+// the workgroup; leaves both buffers idle and slots free on return.
 template <bool THREE>
-__global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
-    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
-    const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
-    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
-    const int32_t *__restrict__ step_s) {
-    // two buffers as two LDS objects: the compiler then knows that the reads
-    // of one do not wait for the direct loads into the other
-    __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
-    __shared__ __attribute__((aligned(1024))) uint16_t lds1[QQ_BUF];
-    __shared__ int4 slots[32];  // split slots of the four tiles of the i-th listed step
-    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
-    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (item >= total) return;
-    const int m = (int)(item / nquads), q = (int)(item % nquads);
+__device__ __forceinline__ void quad_pp_body(int m, int q, float *__restrict__ tiles, int64_t mat_stride, int nt, int g0,
+                                             const int32_t *__restrict__ quads, const int32_t *__restrict__ quad_mask,
+                                             const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
+                                             int64_t split_stride, const int32_t *__restrict__ ppos,
+                                             const int32_t *__restrict__ step_s, uint16_t *lds0, uint16_t *lds1,
+                                             int4 *slots, int tid, int wave, int lane) {
     const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
     const int4 qm = reinterpret_cast<const int4 *>(quad_mask)[q];
@@ -976,32 +990,62 @@ __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
     __syncthreads();
     if (n) {
         const int ua = ia, ub = 2 + jb;
-        qq_issue(lds0, sp, slots[0], 0, wave, lane);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        // K quarters 0 and 2 of listed step i come from lds0, 1 and 3 from
-        // lds1; while one buffer is multiplied, the next quarter lands in the
-        // other (the last step's first quarter again after the end, read by no
-        // wave)
-        for (int i = 0; i < n; ++i) {
-            const uint32_t keep = (work >> i) & 1u ? 0xffffffffu : 0u;
-            const int4 cur = slots[i], nxt = slots[i + 1 < n ? i + 1 : i];
-#pragma unroll
-            for (int q = 0; q < 4; q += 2) {
-                qq_issue(lds1, sp, cur, q + 1, wave, lane);
-                __builtin_amdgcn_sched_barrier(0);  // the loads go out first
-                qq_mma(lds0, ua, ub, cre, cim, lane, THREE, keep);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-                qq_issue(lds0, sp, q == 0 ? cur : nxt, q == 0 ? 2 : 0, wave, lane);
-                __builtin_amdgcn_sched_barrier(0);
-                qq_mma(lds1, ua, ub, cre, cim, lane, THREE, keep);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-            }
+        const int nq = 4 * n;  // K quarters of the listed steps, flattened
+        // quarter g (step g / 4, K quarter g % 4) goes to buffer g % 2; past
+        // the end the last quarter is loaded again, into a buffer no later
+        // quarter reads
+        auto issue = [&](uint16_t *buf, int g) {
+            const int gg = g < nq ? g : nq - 1;
+            qq_issue(buf, sp, slots[gg >> 2], gg & 3, wave, lane);
+        };
+        issue(lds0, 0);
+        issue(lds1, 1);
+        // One quarter: wait for its loads (the next quarter's stay in flight),
+        // barrier (its four tiles are in LDS for every wave), read this wave's
+        // operands into registers, barrier (every wave has read: the buffer is
+        // free), refill the buffer two quarters ahead, multiply.  The loads
+        // thus have two quarters of products to land.  The barriers are the
+        // bare instruction: a __syncthreads() fence would drain the loads in
+        // flight.
+        auto quarter = [&](uint16_t *buf, int g) {
+            __builtin_amdgcn_s_waitcnt(WAIT_VM8);
+            __builtin_amdgcn_s_barrier();
+            const uint32_t keep = (work >> (g >> 2)) & 1u ? 0xffffffffu : 0u;
+            const QuarterOps ops = qq_read(buf, ua, ub, lane, THREE, keep);
+            __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+            __builtin_amdgcn_s_barrier();
+            issue(buf, g + 2);
+            __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products
+            qq_products(ops, cre, cim, THREE);
+        };
+        for (int g = 0; g < nq; g += 2) {
+            quarter(lds0, g);
+            quarter(lds1, g + 1);
         }
+        // the loads issued past the end land before LDS is reused
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+        __syncthreads();
     }
     if (mine) store_acc_tile(C, cre, cim, lane);
+}
+
+template <bool THREE>
+__global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
+    const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
+    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
+    const int32_t *__restrict__ step_s) {
+    // two buffers as two LDS objects: the compiler then knows that the reads
+    // of one do not wait for the direct loads into the other
+    __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
+    __shared__ __attribute__((aligned(1024))) uint16_t lds1[QQ_BUF];
+    __shared__ int4 slots[32];  // split slots of the four tiles of the i-th listed step
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    quad_pp_body<THREE>((int)(item / nquads), (int)(item % nquads), tiles, mat_stride, nt, g0, quads, quad_mask,
+                        tile_id, split, split_stride, ppos, step_s, lds0, lds1, slots, tid, wave, lane);
 }
 
 template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,

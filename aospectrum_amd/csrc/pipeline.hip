@@ -316,9 +316,10 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     if (const char *env = std::getenv("AOSX_SCHUR_GROUP")) group = std::min(32, std::max(1, std::atoi(env)));
     plan_schur(group);
     if (const char *env = std::getenv("AOSX_GRAPHS")) use_graphs = std::atoi(env) != 0;
-    if (const char *env = std::getenv("AOSX_QUAD_PIPELINE")) quad_pipelined = std::atoi(env) != 0;
+    if (const char *env = std::getenv("AOSX_QUAD_PIPELINE")) quad_pipeline = std::min(1, std::max(0, std::atoi(env)));
     peak_bytes = resident_bytes;
     noalloc_scratch = scratch_address(device);
+
 }
 
 // Step groups of `group` consecutive elimination steps and the trailing pair
@@ -334,6 +335,7 @@ void Bundle::plan_schur(int group) {
     grp_tiles.clear();
     std::vector<int32_t> pairs, quads, pair_mask, quad_mask;
     grp_quad_ptr.assign(1, 0);
+
     std::vector<uint8_t> mark((size_t)nt * nt, 0);
     std::vector<int32_t> ppos((size_t)nt * nt, -1), step_s(nt, 0);
     split_tiles = 0;
@@ -395,7 +397,7 @@ void Bundle::plan_schur(int group) {
         }
         auto qmarked = [&](int a, int b) { return a < r && b < r && mark[(size_t)qrows[a] * nt + qrows[b]] != 0; };
         for (int bi = 0; bi < r; bi += 8)
-            for (int bj = 0; bj < r; bj += 8)
+            for (int bj = 0; bj < r; bj += 8) {
                 for (int a = bi; a < std::min(r, bi + 8); a += 2)
                     for (int b = bj; b < std::min(r, bj + 8); b += 2)
                         if (qmarked(a, b) || qmarked(a, b + 1) || qmarked(a + 1, b) || qmarked(a + 1, b + 1)) {
@@ -406,6 +408,7 @@ void Bundle::plan_schur(int group) {
                                 quad_mask.push_back((int32_t)steps_of(q4[e]));
                             }
                         }
+            }
         for (int a = 0; a < r; ++a)
             for (int b = 0; b < r; ++b) mark[(size_t)rows[a] * nt + rows[b]] = 0;
         grp_pair_ptr.push_back((int64_t)pairs.size() / 2);
@@ -679,8 +682,8 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
                 const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
                 const int32_t *qd = d_quads + 4 * grp_quad_ptr[g], *qm = d_quad_mask + 4 * grp_quad_ptr[g];
                 const bool three = precision == AOSX_PRECISION_BF16X3_IR;
-                auto quad = quad_pipelined ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
-                                           : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
+                auto quad = quad_pipeline == 1 ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
+                                               : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
                 quad<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, qd, qm, (int)nq, nmat, d_tile_id, split,
                                                  split_tiles, d_ppos, d_step_s);
                 AOSX_LAUNCHED();

@@ -63,8 +63,8 @@ def schur_kernel(precision: str) -> str:
     if not precision.startswith("bf16"):
         return "k_schur_group"
     three = "true" if precision == "bf16x3_ir" else "false"
-    pipelined = os.environ.get("AOSX_QUAD_PIPELINE", "1") != "0"
-    return f"k_schur_quad_pp<{three}>" if pipelined else f"k_schur_quad_split<{three}>"
+    form = "k_schur_quad_split" if os.environ.get("AOSX_QUAD_PIPELINE", "1") == "0" else "k_schur_quad_pp"
+    return f"{form}<{three}>"
 
 
 def traffic_record(workload: str, precision: str, kernel: str) -> dict | None:

@@ -230,9 +230,9 @@ def _bundle_with_env(bundle, **env):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("precision", ["bf16x3_ir", "bf16_ir"])
-def test_pipelined_quad_updates_match_step_kernel(small2d, precision):
-    """The trailing quad update pipelined by K halves accumulates the same
+@pytest.mark.parametrize("precision,form", [("bf16x3_ir", 1), ("bf16_ir", 1)])
+def test_pipelined_quad_updates_match_step_kernel(small2d, precision, form):
+    """The trailing quad update pipelined by K quarters accumulates the same
     products in the same order as the step-at-a-time kernel: with step groups
     of 4 (three trailing launches per factorisation) both give the same
     energies and residuals."""
@@ -241,7 +241,7 @@ def test_pipelined_quad_updates_match_step_kernel(small2d, precision):
     settings = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision=precision,
                             refinement_steps=1, refinement_passes=2)
     step = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_QUAD_PIPELINE=0)
-    piped = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_QUAD_PIPELINE=1)
+    piped = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_QUAD_PIPELINE=form)
     try:
         piped.kernel_stats(True)
         got, _ = band_solve(piped, settings, interval, kpoints=kpts)
