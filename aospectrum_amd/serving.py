@@ -16,10 +16,13 @@ src/aospectrum/cli.py:561-571); this module is a new caller of the Band path.
   their kernels overlap on the GPU.
 * **Admission by fill memory.**  Before a batch runs, its device memory is
   estimated from the bundle's factor size (resident factorisations of every k
-  and shift, plus vectors) and reserved against ``memory_budget_bytes``;
-  workspaces of idle bundles are released (``aosx_bundle_trim_v1``) when that
-  makes room, otherwise the batch waits for memory to free up.  A single k
-  point that can never fit fails its job.
+  and shift, plus vectors) and reserved against ``memory_budget_bytes``.  When
+  the batch ends its reservation is released; the bundle's workspaces stay
+  allocated as a cache (counted against the budget as ``cached``) so that its
+  next batch reuses them, and the cache of an idle bundle is freed
+  (``aosx_bundle_trim_v1``) as soon as another batch needs the room.  A batch
+  that does not fit waits for memory to free up; a single k point that can
+  never fit fails its job at submission.
 * **Policy for saturated subspaces.**  As in the reference
   (src/aospectrum/band.py:239-243) a k point whose CISS search subspace
   saturates fails its job with ``SolverError``.
@@ -116,7 +119,8 @@ class _Resident:
         self.key = key
         self.native = native
         self.pending: deque[_Item] = deque()
-        self.reserved = 0
+        self.reserved = 0   # bytes held by the batch running now
+        self.cached = 0     # bytes of workspaces kept after the last batch
         self.busy = False
         self.thread: threading.Thread | None = None
 
@@ -225,20 +229,38 @@ class BandServer:
         return take
 
     def _reserve(self, res: _Resident, need: int) -> bool:
-        """Hold the lock; make room for `need` bytes or return False."""
-        used = sum(r.reserved for r in self._bundles.values() if r is not res)
-        if used + need <= self.budget:
-            res.reserved = max(res.reserved, need)
+        """Hold the lock; make room for `need` bytes or return False.
+
+        A bundle's own cache is taken over by its batch (the workspaces only
+        grow, so the batch holds the larger of the two); caches of idle bundles
+        are freed one at a time until the batch fits."""
+        def used() -> int:
+            return sum(r.reserved + r.cached for r in self._bundles.values() if r is not res)
+
+        def take() -> bool:
+            res.reserved = max(need, res.cached)
+            res.cached = 0
             return True
+
+        if used() + max(need, res.cached) <= self.budget:
+            return take()
         for other in self._bundles.values():
-            if other is not res and not other.busy and other.reserved:
+            if other is not res and not other.busy and other.cached:
                 other.native.trim()
-                other.reserved = 0
-                used = sum(r.reserved for r in self._bundles.values() if r is not res)
-                if used + need <= self.budget:
-                    res.reserved = max(res.reserved, need)
-                    return True
+                other.cached = 0
+                if used() + max(need, res.cached) <= self.budget:
+                    return take()
+        if res.cached and used() + need <= self.budget:
+            # the bundle's own cache is larger than this batch needs
+            res.native.trim()
+            res.cached = 0
+            return take()
         return False
+
+    def memory_in_use(self) -> dict[str, tuple[int, int]]:
+        """Per bundle: (bytes reserved by a running batch, bytes cached)."""
+        with self._lock:
+            return {k: (r.reserved, r.cached) for k, r in self._bundles.items()}
 
     def _worker(self, res: _Resident) -> None:
         while True:
@@ -262,6 +284,7 @@ class BandServer:
             self._run(res, batch, settings)
             with self._lock:
                 res.busy = False
+                res.cached, res.reserved = res.reserved, 0
                 self._lock.notify_all()
 
     def _run(self, res: _Resident, batch: list[_Item], settings: CissSettings) -> None:
