@@ -153,6 +153,9 @@ struct Bundle {
     // trailing quad update form (AOSX_QUAD_PIPELINE): 0 the step-at-a-time
     // kernel, 1 K quarters double-buffered through LDS
     int quad_pipeline = 1;
+    // in-group steps of the grouped solves (AOSX_SOLVE_FOLD): 1 one launch per
+    // step group, 0 a diagonal and an update launch per step
+    int solve_fold = 1;
     std::map<std::vector<int64_t>, GraphEntry> graphs;
     std::vector<TimedRange> *ranges = nullptr;  // collecting ranges of the running sequence
     bool capturing = false;

@@ -29,6 +29,16 @@ __device__ __forceinline__ c64 csub(c64 a, c64 b) { return {a.re - b.re, a.im - 
 __device__ __forceinline__ c64 cconj(c64 a) { return {a.re, -a.im}; }
 __device__ __forceinline__ double cabs2(c64 a) { return a.re * a.re + a.im * a.im; }
 
+// Workgroup barrier that orders LDS accesses only: it waits for the wave's
+// own LDS operations (lgkmcnt), not for its global loads, which stay in
+// flight across it (a __syncthreads() fence covers global memory too and
+// waits for every outstanding load).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Per-matrix base of the tile storage: matrices are stored back to back.
 __device__ __forceinline__ float *tile_ptr(float *tiles, int64_t mat_stride, int mat, int32_t id) {
     return tiles + (int64_t)mat * mat_stride + (int64_t)id * TILE_FLOATS;

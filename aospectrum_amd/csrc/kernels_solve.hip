@@ -193,6 +193,19 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     const int m = (int)(mc / ngrp), c0 = (int)(mc % ngrp) * NC;
     const int i = rows[r];
     const int nc = min(NC, nchunk - c0);
+    // the row's tiles of the group's steps (at most 32), looked up at once:
+    // lane j holds the tile id of step g0 + j
+    const int G = g1 - g0;
+    const int32_t lane_id = lane >= G ? -1
+                            : ctrans ? tile_id[(int64_t)(g0 + lane) * nt + i]
+                                     : tile_id[(int64_t)i * nt + g0 + lane];
+    const uint32_t present = (uint32_t)__ballot(lane_id >= 0);
+    auto tile_of = [&](int k) { return __builtin_amdgcn_readlane(lane_id, k - g0); };
+    auto next_k = [&](int k) {  // the first step >= k with a tile, else g1
+        const int j = k - g0;
+        const uint32_t rest = j >= 32 ? 0u : present & (~0u << j);
+        return rest ? g0 + __builtin_ctz(rest) : g1;
+    };
     floatx16 cre[NC], cim[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -200,13 +213,6 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     if (SPLIT) {
         // the next present tile and its first RHS block are loaded into
         // registers while the current products run
-        auto tile_of = [&](int k) {
-            return ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
-        };
-        auto next_k = [&](int k) {
-            while (k < g1 && tile_of(k) < 0) ++k;
-            return k;
-        };
         const float *tm = tiles + (int64_t)m * mat_stride;
         int k = next_k(g0);
         TileBlock tn;
@@ -215,31 +221,37 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
             tn = tile_block_load(tm + (int64_t)tile_of(k) * TILE_FLOATS, tid);
             yn = fetch_split(yslot(ysplit, m, k, nchunk, c0), tid);
         }
+        // The barriers order LDS only, so the loads issued for the next tile
+        // (and this step's other RHS chunks) stay in flight across them and
+        // land while the products run.
         while (k < g1) {
             const int kn = next_k(k + 1);
-            __syncthreads();  // the previous products are done with LDS
+            lds_barrier();  // the previous products are done with LDS
             stage_split_block(tn, As, ctrans != 0, true, ctrans != 0, tid);
             put_split(Bs, yn, tid);
+            u32x32 ym[NC > 1 ? NC - 1 : 1];
+#pragma unroll
+            for (int c = 1; c < NC; ++c)
+                if (c < nc) ym[c - 1] = fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid);
             if (kn < g1) {
                 tn = tile_block_load(tm + (int64_t)tile_of(kn) * TILE_FLOATS, tid);
                 yn = fetch_split(yslot(ysplit, m, kn, nchunk, c0), tid);
             }
-            __syncthreads();
+            lds_barrier();
             mma_split<true>(As, Bs, cre[0], cim[0], wave, lane);
 #pragma unroll
             for (int c = 1; c < NC; ++c) {
                 if (c >= nc) break;
-                __syncthreads();
-                put_split(Bs, fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid), tid);
-                __syncthreads();
+                lds_barrier();
+                put_split(Bs, ym[c - 1], tid);
+                lds_barrier();
                 mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
             }
             k = kn;
         }
     }
-    for (int k = g0; !SPLIT && k < g1; ++k) {
-        const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
-        if (id < 0) continue;
+    for (int k = next_k(g0); !SPLIT && k < g1; k = next_k(k + 1)) {
+        const int32_t id = tile_of(k);
         const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
         __syncthreads();  // the previous products are done with LDS
         if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
@@ -256,6 +268,181 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
 #pragma unroll
     for (int c = 0; c < NC; ++c)
         if (c < nc) store_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
+}
+
+// The steps of one step group [g0, g1) of a sweep in one launch: block
+// (matrix m, chunk c) takes the group's steps in sweep order (ascending when
+// fwd, descending otherwise) and finishes each as
+//   Y_k <- D_k (Y_k - sum_t T_kt Y_t),  t over the group's finished steps,
+// with T_kt tile (k,t) or, with ctrans, the conjugate transpose of tile (t,k),
+// and D_k as in k_solve_diag.  The sum runs over t in the order in which the
+// step-at-a-time sweep (diag launch, then the update launch of the group's
+// rows) would have applied the updates to Y_k, each product accumulated into
+// the same FP32 accumulators, so the results are the same bit for bit; the
+// group's ~2G launches become one.  Y_k stays in the accumulators across its
+// updates.  FP32 MFMA form.
+__global__ void __launch_bounds__(256) k_solve_group(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
+                                                     const float *__restrict__ tiles, int64_t mat_stride,
+                                                     const int32_t *__restrict__ tile_id, int nt,
+                                                     const float *__restrict__ dinv, int g0, int g1, int fwd,
+                                                     int which, int ctrans) {
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const float *tm = tiles + (int64_t)m * mat_stride;
+    const int G = g1 - g0;
+    auto tile_of = [&](int k, int t) { return ctrans ? tile_id[(int64_t)t * nt + k] : tile_id[(int64_t)k * nt + t]; };
+    for (int s = 0; s < G; ++s) {
+        const int k = fwd ? g0 + s : g1 - 1 - s;
+        float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
+        floatx16 cre, cim;
+        load_acc(yk, cre, cim, wave, lane);
+        for (int u = 0; u < s; ++u) {
+            const int t = fwd ? g0 + u : g1 - 1 - u;
+            const int32_t id = tile_of(k, t);
+            if (id < 0) continue;
+            const float *T = tm + (int64_t)id * TILE_FLOATS;
+            __syncthreads();  // the previous products are done with LDS
+            if (ctrans) stage_a_ctrans(T, Are, Aim, tid);
+            else stage_a(T, Are, Aim, tid);
+            stage_b(y + rhs_block(y_mat_stride, m, t, nchunk, c), Bre, Bim, tid);
+            __syncthreads();
+            mma_fp32<true>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        }
+        // D_k times the updated Y_k, read back from memory as the B operand
+        store_acc(yk, cre, cim, wave, lane);
+        __syncthreads();
+        const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+        if (ctrans) stage_a_ctrans(d, Are, Aim, tid);
+        else stage_a(d, Are, Aim, tid);
+        stage_b(yk, Bre, Bim, tid);
+        __syncthreads();
+        cre = floatx16{};
+        cim = floatx16{};
+        mma_fp32<false>(Are, Aim, Bre, Bim, cre, cim, wave, lane);
+        store_acc(yk, cre, cim, wave, lane);
+        __syncthreads();  // Y_k is visible to the whole block
+    }
+}
+
+// Split B operand (planes re_hi, re_lo, im_hi, im_lo; rows = RHS columns,
+// columns = RHS rows, stride SLD) of the wave's accumulator block: the layout
+// stage_split(.., transpose = true, ..) gives for the same FP32 values.
+__device__ __forceinline__ void acc_to_split(uint16_t *S, const floatx16 &cre, const floatx16 &cim, int wave,
+                                             int lane) {
+    const int col = (wave & 1) * 32 + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;
+        uint16_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            split_pair(cre[4 * q + e], rh[e], rl[e]);
+            split_pair(cim[4 * q + e], ih[e], il[e]);
+        }
+        auto pack = [](const uint16_t *v) {
+            return make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
+        };
+        const int off = col * SLD + row;
+        *reinterpret_cast<uint2 *>(S + off) = pack(rh);
+        *reinterpret_cast<uint2 *>(S + SPL + off) = pack(rl);
+        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) = pack(ih);
+        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) = pack(il);
+    }
+}
+
+// The same with split operands (BF16 precisions), as a pipeline of
+// operations: op (s, u) with u < s is the update of step s by finished step u
+// (tile present), op (s, s) the diagonal product of step s.  While one op's
+// products run, the next op's A tile (T_kt or D_k, FP32) and, from its slot,
+// the split Y_t are already loading (the barriers between ops order LDS only);
+// a diagonal op also loads the next step's Y.  The B operand of a diagonal op
+// is the accumulator block written straight to LDS as split values, and the
+// finished Y_k stays in LDS (Ys) for the next step, whose first update most
+// often needs it; the finished Y_k also goes to memory (FP32, and split in its
+// slot for the trailing update).
+__global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict__ y, int64_t y_mat_stride,
+                                                              int nchunk, const float *__restrict__ tiles,
+                                                              int64_t mat_stride, const int32_t *__restrict__ tile_id,
+                                                              int nt, const float *__restrict__ dinv, int g0, int g1,
+                                                              int fwd, int which, int ctrans,
+                                                              uint16_t *__restrict__ ysplit) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL], Ys[4 * SPL];
+    __shared__ int32_t op_id[32 * 32];  // tile of update (s, u), u < s; -1 when absent
+    __shared__ uint32_t op_mask[32];    // bit u of step s: update (s, u) present
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const float *tm = tiles + (int64_t)m * mat_stride;
+    const int G = g1 - g0;  // at most 32
+    auto step_of = [&](int s) { return fwd ? g0 + s : g1 - 1 - s; };
+    // the group's tile lookups, all at once: wave w takes steps w, w + 4, ..
+    for (int s = wave; s < G; s += 4) {
+        const int k = step_of(s), t = step_of(lane);
+        const int32_t id = lane >= s ? -1 : ctrans ? tile_id[(int64_t)t * nt + k] : tile_id[(int64_t)k * nt + t];
+        const uint32_t bits = (uint32_t)__ballot(id >= 0);
+        if (lane < 32) op_id[s * 32 + lane] = id;
+        if (lane == 0) op_mask[s] = bits;
+    }
+    __syncthreads();
+    // the op after (s, u): the next present update of step s, else its
+    // diagonal; after a diagonal, the first op of step s + 1
+    auto advance = [&](int &s, int &u) {
+        if (u == s) {
+            ++s;
+            u = -1;
+        }
+        if (s >= G) return;
+        const uint32_t rest = u < 0 ? op_mask[s] : op_mask[s] & (~0u << (u + 1));
+        u = rest ? __builtin_ctz(rest) : s;
+    };
+    auto a_tile = [&](int s, int u) -> const float * {
+        if (u == s) return dinv + ((int64_t)m * nt + step_of(s)) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+        return tm + (int64_t)op_id[s * 32 + u] * TILE_FLOATS;
+    };
+    int s = 0, u = 0;  // step 0 has no updates: its diagonal comes first
+    TileBlock an = tile_block_load(a_tile(0, 0), tid);
+    u32x32 bn = {};
+    floatx16 cre, cim;
+    load_acc(y + rhs_block(y_mat_stride, m, step_of(0), nchunk, c), cre, cim, wave, lane);
+    while (s < G) {
+        const bool diag = u == s;
+        int s2 = s, u2 = u;
+        advance(s2, u2);
+        lds_barrier();  // the previous products are done with LDS
+        stage_split_block(an, As, ctrans != 0, !diag, ctrans != 0, tid);
+        const uint16_t *B = Bs;
+        if (diag) acc_to_split(Bs, cre, cim, wave, lane);
+        else if (u == s - 1) B = Ys;  // the step finished last
+        else put_split(Bs, bn, tid);
+        // the next op's operands: finished steps other than the last are in
+        // their slots (stored before an earlier full barrier)
+        if (s2 < G) {
+            an = tile_block_load(a_tile(s2, u2), tid);
+            if (u2 < s2 - 1) bn = fetch_split(yslot(ysplit, m, step_of(u2), nchunk, c), tid);
+        }
+        floatx16 nre = {}, nim = {};
+        if (diag && s + 1 < G) load_acc(y + rhs_block(y_mat_stride, m, step_of(s + 1), nchunk, c), nre, nim, wave, lane);
+        lds_barrier();
+        if (diag) {
+            cre = floatx16{};
+            cim = floatx16{};
+        }
+        mma_split<true>(As, B, cre, cim, wave, lane);
+        if (diag) {
+            const int k = step_of(s);
+            store_acc(y + rhs_block(y_mat_stride, m, k, nchunk, c), cre, cim, wave, lane);
+            store_acc_split_t(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
+            // Ys: its last readers (the updates of step s by step s - 1) ran
+            // before this op's first barrier
+            acc_to_split(Ys, cre, cim, wave, lane);
+            __syncthreads();  // the slot stores and Ys are visible to the whole block
+            cre = nre;
+            cim = nim;
+        }
+        s = s2;
+        u = u2;
+    }
 }
 
 template __global__ void k_solve_trail<false, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,

@@ -317,6 +317,7 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     plan_schur(group);
     if (const char *env = std::getenv("AOSX_GRAPHS")) use_graphs = std::atoi(env) != 0;
     if (const char *env = std::getenv("AOSX_QUAD_PIPELINE")) quad_pipeline = std::min(1, std::max(0, std::atoi(env)));
+    if (const char *env = std::getenv("AOSX_SOLVE_FOLD")) solve_fold = std::atoi(env) != 0 ? 1 : 0;
     peak_bytes = resident_bytes;
     noalloc_scratch = scratch_address(device);
 
@@ -451,6 +452,21 @@ void Bundle::plan_schur(int group) {
         }
     }
     crout_ptr[nt] = (int64_t)pairs.size() / 2;
+    if (std::getenv("AOSX_DEBUG")) {
+        // listed steps of the quads (every wave runs the MFMAs of each) and
+        // the steps where a wave's target has an update
+        int64_t listed = 0, work = 0;
+        for (size_t q = 0; q < quad_mask.size() / 4; ++q) {
+            const uint32_t rm[2] = {(uint32_t)quad_mask[4 * q], (uint32_t)quad_mask[4 * q + 1]};
+            const uint32_t cm[2] = {(uint32_t)quad_mask[4 * q + 2], (uint32_t)quad_mask[4 * q + 3]};
+            listed += __builtin_popcount((rm[0] | rm[1]) & (cm[0] | cm[1]));
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) work += __builtin_popcount(rm[a] & cm[b]);
+        }
+        std::fprintf(stderr, "AOSX_DEBUG|plan|group=%d|quads=%zu|listed_steps=%lld|wave_steps=%lld|work_fraction=%.4f\n",
+                     group, quad_mask.size() / 4, (long long)listed, (long long)work,
+                     listed ? (double)work / (4.0 * (double)listed) : 0.0);
+    }
     std::vector<int32_t> frows, brows;
     fwd_row_ptr.assign(1, 0);
     bwd_row_ptr.assign(1, 0);
@@ -730,6 +746,17 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
     };
     const int ngrp = (nchunk + 1) / 2;
     const int ng = (int)grp_begin.size() - 1;
+    // the steps of group [g0, g1) in one launch (see k_solve_group)
+    auto group_launch = [&](int g0, int g1, bool fwd, int which) {
+        const unsigned blocks = (unsigned)(nmat * nchunk);
+        if (split)
+            k_solve_group_split<<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, dinv,
+                                                            g0, g1, fwd ? 1 : 0, which, ctrans ? 1 : 0, ysplit);
+        else
+            k_solve_group<<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, dinv, g0, g1,
+                                                      fwd ? 1 : 0, which, ctrans ? 1 : 0);
+        AOSX_LAUNCHED();
+    };
     auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
         if (nrows == 0) return;
         const int64_t items = nrows * nmat * ngrp;
@@ -767,7 +794,8 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         // group's trailing rows in one launch
         for (int g = 0; grouped && g < ng; ++g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-            for (int k = g0; k < g1; ++k) {
+            if (solve_fold) group_launch(g0, g1, true, ctrans ? 1 : 0);
+            for (int k = g0; !solve_fold && k < g1; ++k) {
                 diag_launch(k, ctrans ? 1 : 0);
                 const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
                 const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
@@ -782,7 +810,8 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         // backward: the same groups in reverse, steps descending
         for (int g = ng - 1; grouped && g >= 0; --g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-            for (int k = g1 - 1; k >= g0; --k) {
+            if (solve_fold) group_launch(g0, g1, false, ctrans ? 0 : 1);
+            for (int k = g1 - 1; !solve_fold && k >= g0; --k) {
                 diag_launch(k, ctrans ? 0 : 1);
                 const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
                 const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];

@@ -259,6 +259,31 @@ def test_pipelined_quad_updates_match_step_kernel(small2d, precision, form):
         step.close()
 
 
+@pytest.mark.parametrize("precision", ["bf16x3_ir", "fp32_ir"])
+def test_folded_group_solves_match_step_launches(small2d, precision):
+    """One launch per step group for the in-group triangular steps applies
+    the same products in the same order as a diagonal and an update launch per
+    step: with step groups of 4 and 32 matrices per solve (grouped sweeps,
+    plain and conjugate-transposed factors) both give identical energies."""
+    kpts = np.array([[0.13, 0.37, 0.0], [0.25, 0.1, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    settings = CissSettings(contour_batch_size=16, conjugate_pairs=True, precision=precision,
+                            refinement_steps=1, refinement_passes=2)
+    step = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_SOLVE_FOLD=0)
+    folded = _bundle_with_env(small2d, AOSX_SCHUR_GROUP=4, AOSX_SOLVE_FOLD=1)
+    try:
+        got, _ = band_solve(folded, settings, interval, kpoints=kpts)
+        ref, _ = band_solve(step, settings, interval, kpoints=kpts)
+        say("folded group solves", precision, [r.energies_ev.size for r in got])
+        for a, b in zip(got, ref):
+            assert a.energies_ev.size == b.energies_ev.size
+            assert np.array_equal(a.energies_ev, b.energies_ev)
+            assert a.maximum_residual_ev == pytest.approx(b.maximum_residual_ev, rel=1e-9)
+    finally:
+        folded.close()
+        step.close()
+
+
 def test_autotune_picks_a_timed_group_and_keeps_results(small2d, small2d_pattern):
     """The autotuner times every candidate step-group size and keeps the
     fastest; solves under each plan match the FP64 oracle."""
