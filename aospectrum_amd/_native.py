@@ -33,7 +33,7 @@ STAGE_NAMES = (
     "native_total",
 )
 
-PRECISION = {"fp32": 0, "fp32_ir": 1, "bf16x3_ir": 2, "bf16_ir": 3}
+PRECISION = {"fp32": 0, "fp32_ir": 1, "bf16x3_ir": 2, "bf16_ir": 3, "per_shift_ir": 4}
 
 # status codes that describe caller input rather than a device failure
 _INPUT_STATUSES = {10, 11, 12, 13, 14, 30, 31, 32, 33, 34, 35, 36, 50, 51, 52, 64}
@@ -195,6 +195,7 @@ class BandRequest(C.Structure):
         ("eigenvectors", _C64P),
         ("pass_rank", _I32P),
         ("pass_width", _I32P),
+        ("far_threshold", C.c_double),
     ]
 
 

@@ -9,7 +9,8 @@ filter and residuals -- runs in the native library; this module validates
 settings, builds the probe block and converts results.
 
 Settings beyond the reference's (all default to the reference's behaviour):
-``precision`` (``fp32`` | ``fp32_ir`` | ``bf16x3_ir`` | ``bf16_ir``),
+``precision`` (``fp32`` | ``fp32_ir`` | ``bf16x3_ir`` | ``bf16_ir`` |
+``per_shift_ir``, with ``far_shift_threshold``),
 ``refinement_steps`` (iterative refinement per shifted solve),
 ``refinement_passes`` (extra contour-filter passes over the Ritz block),
 ``k_batch`` (k points factored together), ``conjugate_pairs`` (solve the
@@ -93,6 +94,12 @@ class CissSettings:
     # this many (0: no limit) and they fit in device memory; otherwise every
     # pass factors each shift group again (streamed shifts)
     resident_limit: int = 0
+    # precision "per_shift_ir": contour point j (angle theta_j) is factored
+    # with one BF16 product per Schur update when |sin theta_j| >= this
+    # threshold -- its distance rv |sin theta_j| from the real axis, where the
+    # spectrum lies, bounds the conditioning of z_j S - H -- and with three
+    # (BF16x3) otherwise; every solve is refined on FP64 residuals
+    far_shift_threshold: float = 0.9
 
     @property
     def factorizations_per_launch(self) -> int:
@@ -118,6 +125,8 @@ class CissSettings:
             raise InputError("refinement_policy must be 'fixed' or 'on_demand'")
         if not self.residual_target_ev > 0.0:
             raise InputError("residual_target_ev must be positive")
+        if not 0.0 <= self.far_shift_threshold <= 2.0:
+            raise InputError("far_shift_threshold must lie in [0, 2]")
 
     @property
     def search_dimension(self) -> int:
@@ -287,6 +296,7 @@ def _band_call(native, settings, interval, kpoints, h_data, s_data, intervals, r
     req.k_batch = settings.k_batch
     req.conjugate_pairs = 1 if settings.conjugate_pairs else 0
     req.resident_limit = settings.resident_limit
+    req.far_threshold = settings.far_shift_threshold
     req.energies_ev = N.ptr(energies, C.c_float)
     req.energy_capacity = cap
     req.energy_counts = N.ptr(counts, C.c_int32)

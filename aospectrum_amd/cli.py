@@ -14,7 +14,7 @@ time to the GPU.
 Extra ``[solver]`` keys (defaults keep the reference's behaviour):
 ``precision``, ``refinement_steps``, ``refinement_passes``, ``k_batch``,
 ``conjugate_pairs``, ``refinement_policy``, ``residual_target_ev``,
-``autotune``, ``resident_limit``.
+``autotune``, ``resident_limit``, ``far_shift_threshold``.
 """
 
 from __future__ import annotations
@@ -132,6 +132,7 @@ _SOLVER_KEYS: tuple[tuple[str, Callable[[Any], Any], Any], ...] = (
     ("residual_target_ev", float, 1.0e-4),
     ("autotune", _boolean, False),
     ("resident_limit", int, 0),
+    ("far_shift_threshold", float, 0.9),
 )
 
 

@@ -374,13 +374,15 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         return fail(res, AOSX_BATCH_DIMENSIONS, 0, "invalid k-point count or k batch");
     if (q->integration_points < 1 || q->contour_batch_size < 1 ||
         q->integration_points % q->contour_batch_size || q->block_size < 1 || q->moment_size < 1 ||
-        q->block_size % 32 || q->block_size * q->moment_size > 256 || q->precision < 0 || q->precision > 3 ||
+        q->block_size % 32 || q->block_size * q->moment_size > 256 || q->precision < 0 || q->precision > 4 ||
         q->refinement_steps < 0 || q->refinement_steps > 16 || q->refinement_passes < 0 ||
         q->refinement_passes > 4 || !(q->lower_ev < q->upper_ev) || !std::isfinite(q->lower_ev) ||
         !std::isfinite(q->upper_ev) || !(q->ellipse_vertical_scale > 0.0))
         return fail(res, AOSX_INVALID_SETTINGS, 0,
                     "CISS sizes must be positive, contour_batch_size must divide integration_points, "
                     "block_size must be a multiple of 32 and block_size*moment_size at most 256");
+    if (q->precision == AOSX_PRECISION_SHIFT_IR && !(q->far_threshold >= 0.0 && q->far_threshold <= 2.0))
+        return fail(res, AOSX_INVALID_SETTINGS, 0, "far_threshold must lie in [0, 2]");
     if (q->intervals)
         for (int64_t k = 0; k < q->n_kpoints; ++k)
             if (!(q->intervals[2 * k] < q->intervals[2 * k + 1]) || !std::isfinite(q->intervals[2 * k]) ||
@@ -423,6 +425,7 @@ int aosx_band_solve_v1(aosx_bundle *bundle, const aosx_band_request_v1 *q, aosx_
         p.resident_limit = q->resident_limit;
         p.intervals = q->intervals;
         p.want_vectors = q->eigenvectors != nullptr;
+        p.far_threshold = q->far_threshold;
         aosx::CissOutput o;
         b.band(p, q->k_fractional, reinterpret_cast<const float2 *>(q->h_data),
                reinterpret_cast<const float2 *>(q->s_data), reinterpret_cast<const float2 *>(q->probes),

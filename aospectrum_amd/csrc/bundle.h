@@ -80,6 +80,7 @@ struct CissParams {
     int resident_limit = 0;  // 0: keep factorisations resident whenever they fit
     const double *intervals = nullptr;  // [nk][2] per-k absolute intervals, or null
     bool want_vectors = false;          // return the selected Ritz vectors
+    double far_threshold = 2.0;         // AOSX_PRECISION_SHIFT_IR: |sin theta_j| >= this -> one BF16 product
 };
 
 struct CissOutput {
@@ -202,8 +203,10 @@ struct Bundle {
     // go to this device's static scratch array (no workspace, no copies).
     void *noalloc_scratch = nullptr;  // device address of that array
     void assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, float2 *d_s);
+    // one_product[m] != 0: matrix m takes one BF16 product per update
+    // (AOSX_PRECISION_SHIFT_IR), else three
     void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
-                float *tiles, float *dinv, int *d_replaced);
+                float *tiles, float *dinv, int *d_replaced, const std::vector<uint8_t> *one_product = nullptr);
     // precision selects the MFMA form of the solve products (BF16 split for
     // the BF16 precisions, FP32 otherwise)
     void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans, int precision);

@@ -427,6 +427,10 @@ __global__ void __launch_bounds__(256) k_schur_group(float *__restrict__ tiles, 
     store_acc(C, cre, cim, wave, lane);
 }
 
+// The BF16 update kernels below run over the matrices listed in `mats` (nmat
+// entries): with per-shift precision the matrices that take one BF16 product
+// and those that take three are updated by launches of their own.
+//
 // Multi-step update from split operands (BF16 precisions): the same pairs
 // and step loop as k_schur_group, with L and U read from the
 // split tiles the panels of the group wrote.  Slot of L(i,t) in the matrix's
@@ -615,13 +619,13 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ pairs,
     const int32_t *__restrict__ pair_mask, int npairs, int nmat, const int32_t *__restrict__ tile_id,
     const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
-    const int32_t *__restrict__ step_s) {
+    const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t total = (int64_t)npairs * nmat, per = (total + 7) / 8;
     const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (item >= total) return;
-    const int m = (int)(item / npairs), q = (int)(item % npairs);
+    const int m = mats[item / npairs], q = (int)(item % npairs);
     const int i = pairs[2 * q], j = pairs[2 * q + 1];
     float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
@@ -662,10 +666,10 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
 
 template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, const int32_t *, const int32_t *,
                                                    int, int, const int32_t *, const uint16_t *, int64_t,
-                                                   const int32_t *, const int32_t *);
+                                                   const int32_t *, const int32_t *, const int32_t *);
 template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *,
                                                     int, int, const int32_t *, const uint16_t *, int64_t,
-                                                    const int32_t *, const int32_t *);
+                                                    const int32_t *, const int32_t *, const int32_t *);
 
 // Trailing update of a step group on 2 x 2 blocks of targets (BF16 precisions):
 // quad q = (i0, i1, j0, j1) (an index -1 is absent); wave w updates the full
@@ -751,13 +755,13 @@ __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
     float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
     const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
     const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
-    const int32_t *__restrict__ step_s) {
+    const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
     __shared__ __attribute__((aligned(16))) uint16_t A0[4 * SPL], A1[4 * SPL], B0[4 * SPL], B1[4 * SPL];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
     const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (item >= total) return;
-    const int m = (int)(item / nquads), q = (int)(item % nquads);
+    const int m = mats[item / nquads], q = (int)(item % nquads);
     const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
     const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
     // step masks: rows (L present), columns (U present), this wave's target
@@ -839,9 +843,9 @@ __global__ void __launch_bounds__(256, 1) k_schur_quad_split(
 }
 
 template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                                 const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+                                                 const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
 template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
 
 
 // Trailing update of a step group on 2 x 2 blocks of targets, pipelined by K
@@ -1034,7 +1038,7 @@ __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
     float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
     const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
     const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
-    const int32_t *__restrict__ step_s) {
+    const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
     // two buffers as two LDS objects: the compiler then knows that the reads
     // of one do not wait for the direct loads into the other
     __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
@@ -1044,13 +1048,13 @@ __global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
     const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
     const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (item >= total) return;
-    quad_pp_body<THREE>((int)(item / nquads), (int)(item % nquads), tiles, mat_stride, nt, g0, quads, quad_mask,
+    quad_pp_body<THREE>(mats[item / nquads], (int)(item % nquads), tiles, mat_stride, nt, g0, quads, quad_mask,
                         tile_id, split, split_stride, ppos, step_s, lds0, lds1, slots, tid, wave, lane);
 }
 
 template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                              const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+                                              const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
 template __global__ void k_schur_quad_pp<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                               const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *);
+                                               const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
 
 }  // namespace aosx

@@ -620,16 +620,35 @@ void Bundle::assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, floa
 
 // ------------------------------------------------------------- factorization
 void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
-                    float *tiles, float *dinv, int *d_replaced) {
+                    float *tiles, float *dinv, int *d_replaced, const std::vector<uint8_t> *one_product) {
     // the H, S values of the current k batch live in the "H"/"S" workspaces
     const float2 *d_h = static_cast<const float2 *>(ws.at("H").ptr);
     const float2 *d_s = static_cast<const float2 *>(ws.at("S").ptr);
     int32_t *d_kidx = buf<int32_t>("f_kidx", nmat);
     float2 *d_z = buf<float2>("f_z", nmat);
-    const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
+    const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR ||
+                            precision == AOSX_PRECISION_SHIFT_IR;
     uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
     AOSX_HIP(hipMemcpyAsync(d_kidx, kidx.data(), nmat * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_z, z.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
+    // BF16 updates: the matrices taking three products (BF16x3) first, then
+    // those taking one (all of them in BF16_IR; the flagged shifts in SHIFT_IR)
+    auto one = [&](int m) {
+        return precision == AOSX_PRECISION_BF16_IR ||
+               (precision == AOSX_PRECISION_SHIFT_IR && one_product && (*one_product)[m] != 0);
+    };
+    std::vector<int32_t> mats;
+    for (int m = 0; m < nmat && split_mode; ++m)
+        if (!one(m)) mats.push_back(m);
+    const int n3 = (int)mats.size();
+    for (int m = 0; m < nmat && split_mode; ++m)
+        if (one(m)) mats.push_back(m);
+    int32_t *d_mats = nullptr;
+    if (split_mode) {
+        d_mats = buf<int32_t>("f_mats", nmat);
+        AOSX_HIP(hipMemcpyAsync(d_mats, mats.data(), nmat * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    }
+    const int n1 = split_mode ? nmat - n3 : 0;
     const int64_t mstride = (int64_t)factor_floats();
     const int nt = ts.n_diag;
     const float pivot_rel = 1.0e-6f;
@@ -641,15 +660,22 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
         const int64_t items = np * nmat;
         const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
         const int32_t *pr = d_pairs + 2 * first, *pm = d_pair_mask + first;
-        if (precision == AOSX_PRECISION_BF16X3_IR)
-            k_schur_group_split<true><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat,
-                                                                  d_tile_id, split, split_tiles, d_ppos, d_step_s);
-        else if (precision == AOSX_PRECISION_BF16_IR)
-            k_schur_group_split<false><<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat,
-                                                                   d_tile_id, split, split_tiles, d_ppos, d_step_s);
-        else
+        if (!split_mode) {
             k_schur_group<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, t0, pr, pm, (int)np, nmat, d_tile_id);
-        AOSX_LAUNCHED();
+            AOSX_LAUNCHED();
+            return;
+        }
+        if (n3 > 0) {
+            k_schur_group_split<true><<<(unsigned)(8 * ((np * n3 + 7) / 8)), 256, 0, stream>>>(
+                tiles, mstride, nt, t0, pr, pm, (int)np, n3, d_tile_id, split, split_tiles, d_ppos, d_step_s, d_mats);
+            AOSX_LAUNCHED();
+        }
+        if (n1 > 0) {
+            k_schur_group_split<false><<<(unsigned)(8 * ((np * n1 + 7) / 8)), 256, 0, stream>>>(
+                tiles, mstride, nt, t0, pr, pm, (int)np, n1, d_tile_id, split, split_tiles, d_ppos, d_step_s,
+                d_mats + n3);
+            AOSX_LAUNCHED();
+        }
     };
     auto launches = [&]() {
         const int64_t count4 = (int64_t)nmat * mstride / 4;
@@ -695,14 +721,17 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             const int r = range_begin(RANGE_SCHUR);
             if (split_mode) {
                 const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
-                const unsigned blocks = (unsigned)(8 * ((nq * nmat + 7) / 8));
                 const int32_t *qd = d_quads + 4 * grp_quad_ptr[g], *qm = d_quad_mask + 4 * grp_quad_ptr[g];
-                const bool three = precision == AOSX_PRECISION_BF16X3_IR;
-                auto quad = quad_pipeline == 1 ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
-                                               : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
-                quad<<<blocks, 256, 0, stream>>>(tiles, mstride, nt, g0, qd, qm, (int)nq, nmat, d_tile_id, split,
-                                                 split_tiles, d_ppos, d_step_s);
-                AOSX_LAUNCHED();
+                for (const bool three : {true, false}) {
+                    const int cnt = three ? n3 : n1;
+                    if (cnt == 0) continue;
+                    auto quad = quad_pipeline == 1 ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
+                                                   : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
+                    quad<<<(unsigned)(8 * ((nq * cnt + 7) / 8)), 256, 0, stream>>>(
+                        tiles, mstride, nt, g0, qd, qm, (int)nq, cnt, d_tile_id, split, split_tiles, d_ppos, d_step_s,
+                        three ? d_mats : d_mats + n3);
+                    AOSX_LAUNCHED();
+                }
             } else {
                 multi_update(g0, g1, grp_pair_ptr[g], np);
             }
@@ -710,14 +739,15 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
         }
     };
     run_sequence({0, nmat, precision, (int64_t)tiles, (int64_t)dinv, (int64_t)d_replaced, (int64_t)split,
-                  (int64_t)d_h, (int64_t)d_s, (int64_t)d_kidx, (int64_t)d_z},
+                  (int64_t)d_h, (int64_t)d_s, (int64_t)d_kidx, (int64_t)d_z, (int64_t)d_mats, n3},
                  launches);
 }
 
 // ------------------------------------------------------------------- solves
 // y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
 void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision) {
-    const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR;
+    const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR ||
+                       precision == AOSX_PRECISION_SHIFT_IR;
     float *y = reinterpret_cast<float *>(y2);
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
@@ -1327,6 +1357,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             std::vector<int32_t> kidx;
             std::vector<int> shift;
             std::vector<float2> z, zc;
+            std::vector<uint8_t> one_product;  // per-shift precision: one BF16 product
             int slot0;
         };
         std::vector<Group> groups(ngroups);
@@ -1339,6 +1370,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                     gr.shift.push_back(j);
                     gr.z.push_back(f2(contours[kb + k].z[j]));
                     gr.zc.push_back(f2(std::conj(contours[kb + k].z[j])));
+                    // the contour point's distance from the real axis is
+                    // rv |sin theta_j|; the point and its conjugate share it
+                    const double th = 2.0 * M_PI * (j + 0.5) / N;
+                    gr.one_product.push_back(std::fabs(std::sin(th)) >= p.far_threshold ? 1 : 0);
                 }
             gr.slot0 = resident ? g * G : 0;
         }
@@ -1370,7 +1405,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             float *tl = d_tiles + (size_t)gr.slot0 * ff;
             float *di = d_dinv + (size_t)gr.slot0 * dif;
             double ta = now();
-            factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0);
+            factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0, &gr.one_product);
             AOSX_HIP(hipStreamSynchronize(stream));
             double tb = now();
             out.stages[2] += tb - ta;
@@ -1460,7 +1495,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                 float *di = d_dinv + (size_t)gr.slot0 * dif;
                 if (!resident) {  // streamed: factor this group again
                     const double ta = now();
-                    factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0);
+                    factor(gsz, gr.kidx, gr.z, p.precision, tl, di, d_repl + gr.slot0, &gr.one_product);
                     AOSX_HIP(hipStreamSynchronize(stream));
                     out.stages[2] += now() - ta;
                     out.factorizations += gsz;

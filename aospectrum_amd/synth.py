@@ -71,6 +71,7 @@ class SynthSpec:
     s_row_bound: float = 0.5
     gap_center: float = 0.0
     gap: float = 0.0
+    gap_fill: float = 0.0
     cluster_spread: float | None = None
     cluster_count: int = 0
 
@@ -111,11 +112,14 @@ SPECS: dict[str, SynthSpec] = {
     "slab52k": SynthSpec(
         "slab52k", (20, 20, 10), 2.4, (48.0, 48.0, 44.0), (True, True, False), 1003, gap=2.75
     ),
+    # serving variants: a wide gap with a thin uniform population of levels
+    # inside it, so that every window of half-width 0.5..2 eV around 0 holds
+    # states (about 14 per eV) and none saturates a block-32 search space
     "slab26k_a": SynthSpec(
-        "slab26k_a", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2001, gap=4.5
+        "slab26k_a", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2001, gap=6.0, gap_fill=0.0062
     ),
     "slab26k_b": SynthSpec(
-        "slab26k_b", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2002, gap=4.5
+        "slab26k_b", (20, 10, 10), 2.4, (48.0, 24.0, 44.0), (True, True, False), 2002, gap=6.0, gap_fill=0.0062
     ),
 }
 
@@ -178,7 +182,13 @@ def _onsite_levels(spec: SynthSpec, rng: np.random.Generator, count: int) -> np.
         pick = rng.integers(0, spec.cluster_count, size=count)
         return centres[pick] + rng.uniform(-0.5, 0.5, size=count) * spec.cluster_spread
     u = rng.uniform(0.0, (hi - lo) - removed, size=count) + lo
-    return np.where(u < a, u, u + removed)
+    levels = np.where(u < a, u, u + removed)
+    if spec.gap_fill > 0.0 and removed > 0.0:
+        # a thin, uniform population inside the removed band (drawn after the
+        # other levels, so specs without it are unchanged)
+        inside = rng.uniform(size=count) < spec.gap_fill
+        levels = np.where(inside, rng.uniform(a, b, size=count), levels)
+    return levels
 
 
 def generate(spec: SynthSpec) -> LocalizedOperatorBundle:

@@ -48,9 +48,9 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0}
+PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0, "per_shift_ir": 2500.0}
 # BF16 split products issued per algorithmic product (hi*hi + hi*lo + lo*hi)
-MFMA_PRODUCTS = {"fp32": 1, "fp32_ir": 1, "bf16x3_ir": 3, "bf16_ir": 1}
+MFMA_PRODUCTS = {"fp32": 1, "fp32_ir": 1, "bf16x3_ir": 3, "bf16_ir": 1, "per_shift_ir": None}
 
 
 def log(*parts):
@@ -60,9 +60,9 @@ def log(*parts):
 def schur_kernel(precision: str) -> str:
     """The trailing-update kernel the library dispatches for `precision`
     (pipeline.hip, Bundle::factor)."""
-    if not precision.startswith("bf16"):
+    if not (precision.startswith("bf16") or precision == "per_shift_ir"):
         return "k_schur_group"
-    three = "true" if precision == "bf16x3_ir" else "false"
+    three = {"bf16x3_ir": "true", "bf16_ir": "false", "per_shift_ir": "true|false"}[precision]
     form = "k_schur_quad_split" if os.environ.get("AOSX_QUAD_PIPELINE", "1") == "0" else "k_schur_quad_pp"
     return f"{form}<{three}>"
 
@@ -272,6 +272,8 @@ def main() -> int:
     ap.add_argument("--k-batch", type=int, default=None, help="k points per batch (default: per workload)")
     ap.add_argument("--plain-kpoints", type=int, default=2)
     ap.add_argument("--precision", default="bf16x3_ir")
+    ap.add_argument("--far-threshold", type=float, default=0.9,
+                    help="per_shift_ir: |sin theta| at and above which a contour point takes one BF16 product")
     ap.add_argument("--ir-steps", type=int, default=2)
     ap.add_argument("--refinement-policy", default="on_demand", choices=["fixed", "on_demand"])
     ap.add_argument("--passes", type=int, default=2)
@@ -337,7 +339,7 @@ def main() -> int:
     K = args.k_batch or k_default
     mixed = CissSettings(block_size=block, contour_batch_size=16, k_batch=K, conjugate_pairs=True,
                          precision=args.precision, refinement_steps=args.ir_steps, refinement_passes=args.passes,
-                         refinement_policy=args.refinement_policy)
+                         refinement_policy=args.refinement_policy, far_shift_threshold=args.far_threshold)
     # plain: the reference's configuration -- one shift per factorisation
     # launch, one k point at a time, FP32 LU with refinement
     plain = CissSettings(block_size=block, precision="fp32_ir", refinement_steps=2, refinement_passes=args.passes)
@@ -454,7 +456,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64" if args.precision.startswith("fp32") else f"complex64+{args.precision.split('_')[0]}",
+        "dtype": {"bf16x3_ir": "complex64+bf16x3", "bf16_ir": "complex64+bf16",
+                  "per_shift_ir": "complex64+bf16/bf16x3 per shift"}.get(args.precision, "complex64"),
         "data": "synthetic (seeded generator, no datasets)",
         "config": {
             "workload": f"{spec.name}: n={bundle.n_orbitals}, {bundle.structure.n_atoms} atoms, "
@@ -464,6 +467,7 @@ def main() -> int:
             "moment_size": 4,
             "k_batch": K,
             "precision": args.precision,
+            **({"far_shift_threshold": args.far_threshold} if args.precision == "per_shift_ir" else {}),
             "ir_steps": args.ir_steps,
             "refinement_policy": args.refinement_policy,
             "filter_passes": args.passes,
@@ -501,7 +505,8 @@ def main() -> int:
             "launches": ks["schur_launches"],
             "avg_launch_ms": ks["schur_ms"] / launches,
             "algorithmic_bytes_per_launch": ks["schur_bytes"] / launches,
-            "mfma_issued_tflops": achieved * MFMA_PRODUCTS[args.precision],
+            "mfma_issued_tflops": None if MFMA_PRODUCTS[args.precision] is None
+            else achieved * MFMA_PRODUCTS[args.precision],
         },
         "kernel_ms": {k: v for k, v in ks.items() if k.endswith("_ms")},
         # streaming stages in achieved algorithmic GB/s (HBM peak 8,000)

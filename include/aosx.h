@@ -79,7 +79,11 @@ enum {
     AOSX_PRECISION_FP32_IR = 1,    /* same LU + iterative refinement on FP64 residuals */
     AOSX_PRECISION_BF16X3_IR = 2,  /* Schur updates as three BF16 MFMA products (split
                                       hi/lo operands, FP32 accumulate) + FP64-residual IR */
-    AOSX_PRECISION_BF16_IR = 3     /* Schur updates in one BF16 MFMA product + IR      */
+    AOSX_PRECISION_BF16_IR = 3,    /* Schur updates in one BF16 MFMA product + IR      */
+    AOSX_PRECISION_SHIFT_IR = 4    /* per contour point (band solves only): one BF16
+                                      product for points far from the real axis
+                                      (|sin theta_j| >= far_threshold), three for
+                                      the others; FP64-residual IR for all          */
 };
 
 /* Fill-reducing ordering of the atom graph. */
@@ -297,6 +301,10 @@ typedef struct {
     int32_t *pass_width;           /* [n_kpoints] out, or NULL: the pass capacity,  */
                                    /* moment rank + augmentation_width (the later  */
                                    /* passes refilter the same span)               */
+    double far_threshold;          /* AOSX_PRECISION_SHIFT_IR: contour point j     */
+                                   /* (angle theta_j = 2 pi (j + 1/2) / N) takes   */
+                                   /* one BF16 product when |sin theta_j| >= this; */
+                                   /* in [0, 2] (above 1: none)                    */
 } aosx_band_request_v1;
 
 #define AOSX_STAGE_COUNT 12

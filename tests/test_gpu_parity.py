@@ -284,6 +284,40 @@ def test_folded_group_solves_match_step_launches(small2d, precision):
         step.close()
 
 
+def test_per_shift_precision_dispatch(small2d, small2d_pattern):
+    """Per-shift precision runs the BF16x3 and the single-BF16 matrices of a
+    batch in launches of their own: threshold 2 (no far point) gives the
+    BF16x3 results and threshold 0 (every point far) the single-BF16 results,
+    bit for bit; a threshold in between matches the FP64 oracle."""
+    kpts = np.array([[0.13, 0.37, 0.0], [0.25, 0.1, 0.0]])
+    interval = EnergyInterval(-2.02, 2.02)
+    common = dict(contour_batch_size=16, k_batch=2, conjugate_pairs=True, refinement_steps=2, refinement_passes=2)
+    nb = NativeBundle(small2d, device=0)
+    try:
+        out = {}
+        for label, st in [
+            ("bf16x3", CissSettings(precision="bf16x3_ir", **common)),
+            ("bf16", CissSettings(precision="bf16_ir", **common)),
+            ("none_far", CissSettings(precision="per_shift_ir", far_shift_threshold=2.0, **common)),
+            ("all_far", CissSettings(precision="per_shift_ir", far_shift_threshold=0.0, **common)),
+            ("half_far", CissSettings(precision="per_shift_ir", far_shift_threshold=0.7, **common)),
+        ]:
+            out[label], _ = band_solve(nb, st, interval, kpoints=kpts)
+            say("per-shift", label, [r.energies_ev.size for r in out[label]],
+                [f"{r.maximum_residual_ev:.2e}" for r in out[label]])
+    finally:
+        nb.close()
+    for a, b in [("none_far", "bf16x3"), ("all_far", "bf16")]:
+        for x, y in zip(out[a], out[b]):
+            assert np.array_equal(x.energies_ev, y.energies_ev)
+            assert x.maximum_residual_ev == pytest.approx(y.maximum_residual_ev, rel=1e-9)
+    for k, r in zip(kpts, out["half_far"]):
+        ref = _oracle_window(small2d, small2d_pattern, k, interval.lower_ev, interval.upper_ev)
+        assert r.energies_ev.size == ref.size
+        assert np.max(np.abs(np.sort(r.energies_ev) - ref)) < 1e-4
+        assert r.maximum_residual_ev <= 1e-4
+
+
 def test_autotune_picks_a_timed_group_and_keeps_results(small2d, small2d_pattern):
     """The autotuner times every candidate step-group size and keeps the
     fastest; solves under each plan match the FP64 oracle."""
@@ -404,25 +438,36 @@ def test_band_solve_needs_all_moments(proxy3d, label):
     assert r.maximum_residual_ev <= 1e-4
 
 
+# path points of the 121-point Y-G-X-M path on both sides of each change of
+# the FP64 count (66 -> 67 at 29/30, 67 -> 68 at 66/67, 68 -> 67 at 69/70)
+BULK3D_GOLDEN_K = (0, 29, 30, 66, 67, 69, 70, 120)
+
+
 def test_bulk3d_matches_golden_inertia_counts():
     """13,000-orbital bulk: in-window counts equal the FP64 Sylvester-inertia
-    counts committed in tests/golden/bulk3d_counts.json, residual <= 1e-4 eV."""
+    counts committed in tests/golden/bulk3d_counts.json (all 121 path points;
+    checked here at the points where the count changes, covering counts 66,
+    67 and 68), residual <= 1e-4 eV."""
     import json
     from pathlib import Path
 
     golden = json.loads((Path(__file__).parent / "golden" / "bulk3d_counts.json").read_text())
+    assert golden["k_index"] == list(range(121))
     bundle = synth.generate(synth.SPECS["bulk3d"])
     lo, hi = golden["interval_ev"]
-    settings = CissSettings(contour_batch_size=16, k_batch=3, conjugate_pairs=True, precision="bf16x3_ir",
+    kp = np.asarray([golden["kpoints"][i] for i in BULK3D_GOLDEN_K])
+    want = [golden["counts"][i] for i in BULK3D_GOLDEN_K]
+    assert {66, 67, 68} <= set(want)
+    settings = CissSettings(contour_batch_size=16, k_batch=4, conjugate_pairs=True, precision="bf16x3_ir",
                             refinement_steps=2, refinement_passes=2)
-    say("bulk3d golden counts", golden["counts"])
+    say("bulk3d golden counts", dict(zip(BULK3D_GOLDEN_K, want)))
     nb = NativeBundle(bundle, device=0)
     try:
-        results, _ = band_solve(nb, settings, EnergyInterval(lo, hi), kpoints=np.asarray(golden["kpoints"]))
+        results, _ = band_solve(nb, settings, EnergyInterval(lo, hi), kpoints=kp)
     finally:
         nb.close()
-    for r, count in zip(results, golden["counts"]):
-        say(f"  count={r.energies_ev.size} golden={count} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e}")
+    for i, r, count in zip(BULK3D_GOLDEN_K, results, want):
+        say(f"  k={i} count={r.energies_ev.size} golden={count} rank={r.subspace_rank} res={r.maximum_residual_ev:.2e}")
         assert r.energies_ev.size == count
         assert r.maximum_residual_ev <= 1e-4
 
@@ -442,6 +487,12 @@ def stress2d():
         ("fp32_fp64ir", CissSettings(precision="fp32_ir", refinement_steps=2, refinement_passes=2), True),
         ("bf16x3_fp64ir", CissSettings(contour_batch_size=16, k_batch=2, conjugate_pairs=True,
                                        precision="bf16x3_ir", refinement_steps=2, refinement_passes=2), True),
+        # one BF16 product for the half of the contour farthest from the real
+        # axis (|sin theta| >= 0.7: 8 of the 16 conjugate pairs), BF16x3 for
+        # the rest
+        ("per_shift_fp64ir", CissSettings(contour_batch_size=16, k_batch=2, conjugate_pairs=True,
+                                          precision="per_shift_ir", far_shift_threshold=0.7, refinement_steps=2,
+                                          refinement_passes=2), True),
     ],
 )
 def test_precision_stress_against_fp64(stress2d, label, settings, refined):

@@ -177,3 +177,7 @@ def test_band_solve_settings_validation(small2d):
         CissSettings(precision="fp8")
     with pytest.raises(InputError):
         CissSettings(integration_points=31, conjugate_pairs=True)
+    with pytest.raises(InputError):
+        CissSettings(precision="per_shift_ir", far_shift_threshold=2.5)
+    assert CissSettings(precision="per_shift_ir", far_shift_threshold=0.7).far_shift_threshold == 0.7
+
