@@ -463,9 +463,17 @@ void Bundle::plan_schur(int group) {
             for (int a = 0; a < 2; ++a)
                 for (int b = 0; b < 2; ++b) work += __builtin_popcount(rm[a] & cm[b]);
         }
-        std::fprintf(stderr, "AOSX_DEBUG|plan|group=%d|quads=%zu|listed_steps=%lld|wave_steps=%lld|work_fraction=%.4f\n",
-                     group, quad_mask.size() / 4, (long long)listed, (long long)work,
-                     listed ? (double)work / (4.0 * (double)listed) : 0.0);
+        // in-group solve operations: a diagonal product per step and an
+        // update per tile below the diagonal inside the step's group
+        int64_t ingroup = 0;
+        for (size_t g = 0; g + 1 < grp_begin.size(); ++g)
+            for (int t = grp_begin[g]; t < grp_begin[g + 1]; ++t)
+                for (int32_t a = ts.next_ptr[t]; a < ts.next_ptr[t + 1]; ++a) ingroup += ts.next_idx[a] < grp_begin[g + 1];
+        std::fprintf(stderr,
+                     "AOSX_DEBUG|plan|group=%d|groups=%zu|quads=%zu|listed_steps=%lld|wave_steps=%lld|"
+                     "work_fraction=%.4f|ingroup_updates=%lld|steps=%d\n",
+                     group, grp_begin.size() - 1, quad_mask.size() / 4, (long long)listed, (long long)work,
+                     listed ? (double)work / (4.0 * (double)listed) : 0.0, (long long)ingroup, nt);
     }
     std::vector<int32_t> frows, brows;
     fwd_row_ptr.assign(1, 0);
