@@ -968,8 +968,11 @@ struct Subspace {
 // (leading dimension ld, `stride` elements apart): ascending eigenvalues into
 // w[k][*], eigenvectors as the columns of Vout (rows and columns < rk[k]).
 // A is left unchanged.
+// tol: a sweep stops the iteration once the off-diagonal Frobenius norm is
+// below tol times the total (the rounding floor of an r x r FP64 matrix is
+// about r * 1e-16)
 static void eigh_batched(Bundle &B, const c64 *A, int ld, int64_t stride, const int32_t *d_rk, int rmax, int nk,
-                         c64 *Vout, double *w) {
+                         c64 *Vout, double *w, double tol_default) {
     if (rmax > EIGH_MAX) throw Error(AOSX_SEARCH_DIMENSION, "projected problem exceeds 256 columns");
     hipStream_t st = B.stream;
     const int rr = rmax + (rmax & 1);
@@ -977,15 +980,24 @@ static void eigh_batched(Bundle &B, const c64 *A, int ld, int64_t stride, const 
     Rot *log = B.buf<Rot>("e_log", (size_t)nk * log_stride);
     int32_t *rounds = B.buf<int32_t>("e_rounds", nk);
     int32_t *rank = B.buf<int32_t>("e_rank", (size_t)nk * EIGH_MAX);
+    const double tol = tol_default;
     if (rr <= EIGH_LDS_MAX) {
         k_jacobi_rotate<true><<<nk, 1024, 0, st>>>(A, ld, stride, d_rk, nullptr, log, log_stride, rounds, w, rank,
-                                                   EIGH_SWEEPS, 1e-15);
+                                                   EIGH_SWEEPS, tol);
     } else {
         c64 *scratch = B.buf<c64>("e_packed", (size_t)nk * (EIGH_MAX * (EIGH_MAX + 1) / 2));
         k_jacobi_rotate<false><<<nk, 1024, 0, st>>>(A, ld, stride, d_rk, scratch, log, log_stride, rounds, w, rank,
-                                                    EIGH_SWEEPS, 1e-15);
+                                                    EIGH_SWEEPS, tol);
     }
     AOSX_LAUNCHED();
+    if (std::getenv("AOSX_DEBUG")) {
+        std::vector<int32_t> hr(nk);
+        AOSX_HIP(hipMemcpyAsync(hr.data(), rounds, nk * 4, hipMemcpyDeviceToHost, st));
+        AOSX_HIP(hipStreamSynchronize(st));
+        std::fprintf(stderr, "AOSX_DEBUG|jacobi|rmax=%d|sweeps:", rmax);
+        for (int k = 0; k < nk; ++k) std::fprintf(stderr, " %d", hr[k] / std::max(1, rr - 1));
+        std::fprintf(stderr, "\n");
+    }
     k_jacobi_vectors<<<dim3((rmax + EIGH_ROWS - 1) / EIGH_ROWS, nk), 256, 0, st>>>(Vout, ld, stride, d_rk, log,
                                                                                     log_stride, rounds, rank);
     AOSX_LAUNCHED();
@@ -1023,7 +1035,11 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     std::vector<int32_t> rP(nk, P);
     int32_t *d_rP = B.buf<int32_t>("c_rP", nk);
     AOSX_HIP(hipMemcpyAsync(d_rP, rP.data(), nk * 4, hipMemcpyHostToDevice, st));
-    eigh_batched(B, d_G, P, (int64_t)P * P, d_rP, P, nk, d_U, d_w);
+    // the Gram serves a rank cut at rank_tol (1e-5) and the basis of the
+    // retained directions, applied to complex64 data: 1e-10 is ample, and it
+    // halves the sweeps of 1e-15, which the rounding floor of a 128 x 128
+    // matrix barely admits
+    eigh_batched(B, d_G, P, (int64_t)P * P, d_rP, P, nk, d_U, d_w, 1e-10);
     std::vector<double> w((size_t)nk * P);
     AOSX_HIP(hipMemcpyAsync(w.data(), d_w, w.size() * 8, hipMemcpyDeviceToHost, st));
     AOSX_HIP(hipStreamSynchronize(st));
@@ -1124,7 +1140,7 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
     double *d_E = B.buf<double>("c_E", (size_t)nk * R);
     k_fill_zero<<<blocks_for(nk * RR, 256, 4096), 256, 0, st>>>(reinterpret_cast<float4 *>(d_Y), nk * RR);
     AOSX_LAUNCHED();
-    eigh_batched(B, d_Wm, R, RR, d_rk, rmax, nk, d_Y, d_E);
+    eigh_batched(B, d_Wm, R, RR, d_rk, rmax, nk, d_Y, d_E, 1e-13);  // energies to ~1e-12 eV, r <= 256
     k_small_gemm<<<sg, 64, 0, st>>>(d_C, d_Li, 1, d_Y, 0, d_rk, R, RR);
     AOSX_LAUNCHED();
     double *d_score = B.buf<double>("c_score", (size_t)nk * R);
