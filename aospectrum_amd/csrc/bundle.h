@@ -205,6 +205,8 @@ struct Bundle {
     void assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, float2 *d_s);
     // one_product[m] != 0: matrix m takes one BF16 product per update
     // (AOSX_PRECISION_SHIFT_IR), else three
+    void residual_norms(double *norm2, const double2 *V, int cap, const int32_t *d_count, const double *d_e,
+                        const float2 *d_H, const float2 *d_S, int nk, hipStream_t st);
     void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
                 float *tiles, float *dinv, int *d_replaced, const std::vector<uint8_t> *one_product = nullptr);
     // precision selects the MFMA form of the solve products (BF16 split for

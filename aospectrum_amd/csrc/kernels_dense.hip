@@ -456,4 +456,114 @@ __global__ void __launch_bounds__(256) k_eig_residual(double *__restrict__ norm2
     }
 }
 
+// The same residual on the atom-block structure (every orbital row of an atom
+// has the same column list): workgroup (k, atom) stages the atom's H and S
+// values of 64 columns at a time in LDS (FP64), and each lane -- one vector
+// column c -- gathers V(j, c) once for all of the atom's rows and accumulates
+// (H - e_c S) V over them.  The 8 half-waves split the column list as in
+// k_atom_spmm and are reduced through LDS; per column one FP64 atomic add of
+// the atom's sum of squares.  V is gathered once per atom instead of once per
+// row.  Grid: 8 * ceil(n_atoms * nk / 8) blocks; block b takes item
+// (b % 8) * per + b / 8 = (k, atom), so each XCD walks a contiguous run of one
+// k point's atoms.
+template <int MAXR>
+__global__ void __launch_bounds__(256) k_eig_residual_atoms(
+    double *__restrict__ norm2, const double2 *__restrict__ V, int cap, const int32_t *__restrict__ count,
+    const double *__restrict__ energies, const float2 *__restrict__ h, const float2 *__restrict__ s,
+    const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices, const int32_t *__restrict__ atom_off,
+    int64_t n, int64_t nnz, int n_atoms, int nk) {
+    constexpr int CH = 64, NS = 8;
+    __shared__ double2 hv[CH][MAXR], sv[CH][MAXR];
+    __shared__ double2 red[MAXR][32];
+    __shared__ int32_t cidx[CH];
+    const int64_t total = (int64_t)n_atoms * nk, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int k = (int)(item / n_atoms), a = (int)(item % n_atoms);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, cl = lane & 31, stream = wave * 2 + half;
+    const int r0 = atom_off[a], na = atom_off[a + 1] - r0;
+    const int p0 = indptr[r0], len = indptr[r0 + 1] - p0;
+    const int cnt = count[k];
+    const float2 *hk = h + (int64_t)k * nnz, *sk = s + (int64_t)k * nnz;
+    const double2 *vk = V + (int64_t)k * n * cap;
+    for (int c0 = 0; c0 < cnt; c0 += 32) {
+        const int c = c0 + cl;
+        const bool cv = c < cnt;
+        const double e = cv ? energies[(int64_t)k * cap + c] : 0.0;
+        double ar[MAXR], ai[MAXR];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) ar[r] = ai[r] = 0.0;
+        for (int q0 = 0; q0 < len; q0 += CH) {
+            const int qc = min(CH, len - q0);
+            __syncthreads();
+            for (int x = tid; x < qc * na; x += 256) {
+                const int pp = x / na, r = x % na;
+                const int64_t pos = indptr[r0 + r] + q0 + pp;
+                const float2 hvv = hk[pos], svv = sk[pos];
+                hv[pp][r] = make_double2(hvv.x, hvv.y);
+                sv[pp][r] = make_double2(svv.x, svv.y);
+            }
+            for (int x = tid; x < qc; x += 256) cidx[x] = indices[p0 + q0 + x];
+            __syncthreads();
+            double2 xv[CH / NS];
+#pragma unroll
+            for (int u = 0; u < CH / NS; ++u) {
+                const int pp = stream + NS * u;
+                xv[u] = (cv && pp < qc) ? vk[(int64_t)cidx[pp] * cap + c] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < CH / NS; ++u) {
+                const int pp = stream + NS * u;
+                if (pp >= qc) break;
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r) {
+                    if (r < na) {
+                        const double2 hh = hv[pp][r], ss = sv[pp][r];
+                        const double vr = hh.x - e * ss.x, vi = hh.y - e * ss.y;
+                        ar[r] += vr * xv[u].x - vi * xv[u].y;
+                        ai[r] += vr * xv[u].y + vi * xv[u].x;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            ar[r] += __shfl_down(ar[r], 32);
+            ai[r] += __shfl_down(ai[r], 32);
+        }
+        for (int src = 1; src < 4; ++src) {
+            __syncthreads();
+            if (wave == src && half == 0) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (r < na) red[r][cl] = make_double2(ar[r], ai[r]);
+            }
+            __syncthreads();
+            if (wave == 0 && half == 0) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (r < na) {
+                        ar[r] += red[r][cl].x;
+                        ai[r] += red[r][cl].y;
+                    }
+            }
+        }
+        if (wave == 0 && half == 0 && cv) {
+            double local = 0.0;
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r)
+                if (r < na) local += ar[r] * ar[r] + ai[r] * ai[r];
+            atomicAdd(norm2 + (int64_t)k * cap + c, local);
+        }
+    }
+}
+
+template __global__ void k_eig_residual_atoms<16>(double *, const double2 *, int, const int32_t *, const double *,
+                                                  const float2 *, const float2 *, const int32_t *, const int32_t *,
+                                                  const int32_t *, int64_t, int64_t, int, int);
+template __global__ void k_eig_residual_atoms<32>(double *, const double2 *, int, const int32_t *, const double *,
+                                                  const float2 *, const float2 *, const int32_t *, const int32_t *,
+                                                  const int32_t *, int64_t, int64_t, int, int);
+
 }  // namespace aosx

@@ -923,6 +923,23 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
 }
 
 // -------------------------------------------------------- eigenpair residual
+// norm2[k][c] += ||(H_k - e_c S_k) v_c||^2 for the count[k] columns of V[k]
+// (FP64): the atom-block form when atoms have at most 32 orbitals, else the
+// row form.
+void Bundle::residual_norms(double *norm2, const double2 *V, int cap, const int32_t *d_count, const double *d_e,
+                            const float2 *d_H, const float2 *d_S, int nk, hipStream_t st) {
+    if (max_atom <= 32) {
+        const unsigned blocks = (unsigned)(8 * ((n_atoms * nk + 7) / 8));
+        auto kern = max_atom <= 16 ? k_eig_residual_atoms<16> : k_eig_residual_atoms<32>;
+        kern<<<blocks, 256, 0, st>>>(norm2, V, cap, d_count, d_e, d_H, d_S, d_indptr, d_indices, d_atom_off, n, nnz,
+                                     (int)n_atoms, nk);
+    } else {
+        k_eig_residual<<<dim3(blocks_for(n, 4, 1024), (unsigned)nk), 256, 0, st>>>(norm2, V, cap, d_count, d_e, d_H,
+                                                                                  d_S, d_indptr, d_indices, n, nnz);
+    }
+    AOSX_LAUNCHED();
+}
+
 void Bundle::eig_residual(int64_t nk, const double *kpts, const float2 *h_in, const float2 *s_in, int count,
                           const double *energies, const double *vectors, double *norm) {
     AOSX_HIP(hipSetDevice(device));
@@ -943,9 +960,7 @@ void Bundle::eig_residual(int64_t nk, const double *kpts, const float2 *h_in, co
     AOSX_HIP(hipMemcpyAsync(d_e, energies, (size_t)nk * count * 8, hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_cnt, cnt.data(), nk * 4, hipMemcpyHostToDevice, stream));
     zero(d_n2, ((size_t)nk * count * 8 + 15) / 16 * 16);
-    k_eig_residual<<<dim3(blocks_for(n, 4, 1024), (unsigned)nk), 256, 0, stream>>>(d_n2, d_vec, count, d_cnt, d_e, d_H,
-                                                                                  d_S, d_indptr, d_indices, n, nnz);
-    AOSX_LAUNCHED();
+    residual_norms(d_n2, d_vec, count, d_cnt, d_e, d_H, d_S, nk, stream);
     std::vector<double> n2((size_t)nk * count);
     AOSX_HIP(hipMemcpyAsync(n2.data(), d_n2, n2.size() * 8, hipMemcpyDeviceToHost, stream));
     AOSX_HIP(hipStreamSynchronize(stream));
@@ -1209,9 +1224,7 @@ static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, co
         double *d_n2 = B.buf<double>("r_norm2", (size_t)nk * Mc + 2);
         const int64_t z4 = ((int64_t)nk * Mc * 8 + 15) / 16;
         k_fill_zero<<<blocks_for(z4, 256, 1024), 256, 0, st>>>(reinterpret_cast<float4 *>(d_n2), z4);
-        k_eig_residual<<<dim3(blocks_for(n, 4, 1024), nk), 256, 0, st>>>(d_n2, d_vec, Mc, d_scnt, d_sen, d_H, d_S,
-                                                                       B.d_indptr, B.d_indices, n, nnz);
-        AOSX_LAUNCHED();
+        B.residual_norms(d_n2, d_vec, Mc, d_scnt, d_sen, d_H, d_S, nk, st);
         std::vector<double> n2((size_t)nk * Mc);
         AOSX_HIP(hipMemcpyAsync(n2.data(), d_n2, n2.size() * 8, hipMemcpyDeviceToHost, st));
         std::vector<double2> hv;
