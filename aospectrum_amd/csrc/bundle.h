@@ -151,8 +151,8 @@ struct Bundle {
     // the first call with a given key (sizes, precision, buffer addresses)
     // and replayed afterwards; AOSX_GRAPHS=0 runs them eagerly.
     bool use_graphs = true;
-    // trailing quad update form (AOSX_QUAD_PIPELINE): 0 the step-at-a-time
-    // kernel, 1 K quarters double-buffered through LDS
+    // trailing update form (AOSX_QUAD_PIPELINE): 0 one target per workgroup
+    // (the reference form), 1 2 x 2 quads staged through LDS
     int quad_pipeline = 1;
     // in-group steps of the grouped solves (AOSX_SOLVE_FOLD): 1 one launch per
     // step group, 0 a diagonal and an update launch per step

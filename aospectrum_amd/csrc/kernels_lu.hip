@@ -24,7 +24,8 @@
 // panel and Schur products split every operand into hi + lo BF16 parts for
 // v_mfma_f32_32x32x16_bf16 (three products, FP32 accumulation), the panels
 // also write their tiles in that split form for the group's updates, and the
-// stored factor stays FP32.
+// stored factor stays FP32.  The trailing update forms each complex product
+// from three real ones (re*re, im*im, (re+im)*(re+im)) instead of four.
 #include "device_common.h"
 
 namespace aosx {
@@ -300,18 +301,14 @@ __device__ __forceinline__ void store_acc(float *__restrict__ t, const floatx16 
     }
 }
 
-// Split copies of panel tiles for the BF16 trailing update.  A split tile is
-// four planes of 64 x 64 BF16 -- re_hi, re_lo, im_hi, im_lo, with
-// x ~ hi + lo, hi = bf16(x), lo = bf16(x - hi) (round to nearest even) --
-// holding -L(i,k) row-major ([row][col]) or U(k,j) transposed ([col][row]), so
-// that both MFMA operands read 8 consecutive k values per lane.  The tile is
-// stored in MFMA operand order: 1 KiB blocks, one per (K half, plane, 32-row
-// half a, 16-k step within the half), each holding the 32 x 16 operand of one
-// v_mfma_f32_32x32x16_bf16 as lane l of the wave reads it -- row 32a + l % 32,
-// k values 8 (l / 32) .. + 7 -- in lane order.  A K half of all four planes is
-// 16 KiB of consecutive memory, and a wave reads an operand block from LDS
-// as one linear, bank-conflict-free 16-byte-per-lane read.
-constexpr int SPLIT_TILE = 4 * TILE2;  // BF16 elements per split tile (32 KiB)
+// Split RHS blocks of the solves (BF16 precisions): four planes of 64 x 64
+// BF16 -- re_hi, re_lo, im_hi, im_lo, with x ~ hi + lo, hi = bf16(x),
+// lo = bf16(x - hi) (round to nearest even) -- stored in MFMA operand order:
+// 1 KiB blocks, one per (K half, plane, 32-row half a, 16-k step within the
+// half), each holding the 32 x 16 operand of one v_mfma_f32_32x32x16_bf16 as
+// lane l of the wave reads it -- row 32a + l % 32, k values 8 (l / 32) .. + 7
+// -- in lane order.
+constexpr int SPLIT_TILE = 4 * TILE2;  // BF16 elements per split RHS block (32 KiB)
 constexpr int SPLIT_HPLANE = TILE * 32;      // one plane of one K half (4 blocks)
 constexpr int SPLIT_HALF = 4 * SPLIT_HPLANE; // all planes of one K half (16 blocks)
 constexpr int SPLIT_BLOCK = 512;             // BF16 per operand block
@@ -327,14 +324,33 @@ __device__ __forceinline__ void split_pair(float x, uint16_t &hi, uint16_t &lo) 
     lo = __builtin_bit_cast(uint16_t, l);
 }
 
-// Writes the tile held in LDS (re/im planes, row stride LDA) as a split tile.
-// negate_rows: -T row-major (the L operand); otherwise T transposed (the U
-// operand).  Thread tid covers 16 consecutive destination elements of one
-// destination row, written as 2 x 16 B per plane.
-__device__ __forceinline__ void write_split(const float *Tre, const float *Tim, uint16_t *__restrict__ dst,
+// Split panel tiles (BF16 precisions).  The panels of a step group keep each
+// tile they finish, for the group's Schur updates, as six planes of 64 x 64
+// BF16: the real part, the imaginary part and their sum re + im (formed in
+// FP32), each as a hi and a lo part, of -L(i,k) row-major ([row][col]) or of
+// U(k,j) transposed ([col][row]), so that both MFMA operands read 8
+// consecutive k values per lane.  The sum planes serve the three-
+// multiplication complex product (Gauss):
+//   P U = (Pr Ur - Pi Ui) + i ((Pr + Pi)(Ur + Ui) - Pr Ur - Pi Ui),
+// three real products where the textbook form takes four.  The tile is stored
+// in MFMA operand order: 1 KiB blocks, block ((q * 3 + c) * 2 + h) * 2 + a
+// for K quarter q, component c (re, im, sum), part h (hi, lo) and 32-row half
+// a, each holding the 32 x 16 operand of one v_mfma_f32_32x32x16_bf16 in lane
+// order (lane l: row 32a + l % 32, k values 16q + 8 (l / 32) .. + 7).  One
+// (quarter, component) is 4 KiB of consecutive memory, and a wave reads a
+// block from LDS as one linear, bank-conflict-free 16-byte-per-lane read.
+constexpr int PANEL_TILE = 6 * TILE2;  // BF16 elements per split panel tile (48 KiB)
+__device__ __forceinline__ int panel_block(int q, int c, int h, int a) { return ((q * 3 + c) * 2 + h) * 2 + a; }
+
+// Writes the tile held in LDS (re/im planes, row stride LDA) as a split panel
+// tile.  negate_rows: -T row-major (the L operand); otherwise T transposed
+// (the U operand).  Thread tid covers the 16 destination elements of one
+// destination row in K quarter tid % 4: per plane, 8 k values in each of the
+// two lane halves of one block (2 x 16 B).
+__device__ __forceinline__ void write_panel(const float *Tre, const float *Tim, uint16_t *__restrict__ dst,
                                             bool negate_rows, int tid) {
-    const int r = tid >> 2, c0 = (tid & 3) * 16;
-    uint16_t rh[16], rl[16], ih[16], il[16];
+    const int r = tid >> 2, q = tid & 3, c0 = 16 * q;
+    uint16_t v[6][16];  // [component * 2 + part][k]
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         float xr, xi;
@@ -345,20 +361,19 @@ __device__ __forceinline__ void write_split(const float *Tre, const float *Tim, 
             xr = Tre[(c0 + c) * LDA + r];
             xi = Tim[(c0 + c) * LDA + r];
         }
-        split_pair(xr, rh[c], rl[c]);
-        split_pair(xi, ih[c], il[c]);
+        split_pair(xr, v[0][c], v[1][c]);
+        split_pair(xi, v[2][c], v[3][c]);
+        split_pair(xr + xi, v[4][c], v[5][c]);
     }
-    // k = c0 .. c0 + 7 and c0 + 8 .. c0 + 15: two 8-element runs of one block
-    const int off = split_index(0, r, c0), off8 = split_index(0, r, c0 + 8);
-    auto pack = [](const uint16_t *v, int o) {
-        return make_uint4(v[o] | ((uint32_t)v[o + 1] << 16), v[o + 2] | ((uint32_t)v[o + 3] << 16),
-                          v[o + 4] | ((uint32_t)v[o + 5] << 16), v[o + 6] | ((uint32_t)v[o + 7] << 16));
+    auto pack = [](const uint16_t *w) {
+        return make_uint4(w[0] | ((uint32_t)w[1] << 16), w[2] | ((uint32_t)w[3] << 16),
+                          w[4] | ((uint32_t)w[5] << 16), w[6] | ((uint32_t)w[7] << 16));
     };
-    const uint16_t *planes[4] = {rh, rl, ih, il};
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) {
-        *reinterpret_cast<uint4 *>(dst + pl * SPLIT_HPLANE + off) = pack(planes[pl], 0);
-        *reinterpret_cast<uint4 *>(dst + pl * SPLIT_HPLANE + off8) = pack(planes[pl], 8);
+    for (int p = 0; p < 6; ++p) {
+        uint16_t *b = dst + panel_block(q, p >> 1, p & 1, r >> 5) * SPLIT_BLOCK + (r & 31) * 8;
+        *reinterpret_cast<uint4 *>(b) = pack(v[p]);            // k 16q + 0..7: lanes 0..31
+        *reinterpret_cast<uint4 *>(b + 256) = pack(v[p] + 8);  // k 16q + 8..15: lanes 32..63
     }
 }
 
@@ -581,13 +596,14 @@ __global__ void __launch_bounds__(256, 2) k_panel_split(float *__restrict__ tile
         Tim[row * LDA + c] = cim[r];
     }
     __syncthreads();
-    uint16_t *dst = split + ((int64_t)m * split_stride + slot0 + blockIdx.x) * SPLIT_TILE;
-    write_split(Tre, Tim, dst, lower, tid);
+    uint16_t *dst = split + ((int64_t)m * split_stride + slot0 + blockIdx.x) * PANEL_TILE;
+    write_panel(Tre, Tim, dst, lower, tid);
 }
 
-// One split tile = 2048 chunks of 16 B; thread tid moves chunks q * 256 + tid.
-// The chunks travel in a register vector (not an array, which the compiler
-// may place in scratch memory when it is carried across loop iterations).
+// One split RHS block = 2048 chunks of 16 B; thread tid moves chunks
+// q * 256 + tid.  The chunks travel in a register vector (not an array, which
+// the compiler may place in scratch memory when it is carried across loop
+// iterations).
 typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
 
 __device__ __forceinline__ u32x32 fetch_split(const uint16_t *__restrict__ src, int tid) {
@@ -614,6 +630,37 @@ __device__ __forceinline__ void put_split(uint16_t *dst, const u32x32 r, int tid
     }
 }
 
+// The re and im planes of a split panel tile (32 KiB of its 48): 2048 chunks
+// of 16 B, chunk e being lane e % 64 of the re/im block e / 64, whose bits
+// are (K quarter, component, part, row half) from high to low.  Thread tid
+// moves chunks q * 256 + tid, so each wave reads whole blocks.
+__device__ __forceinline__ u32x32 fetch_panel4(const uint16_t *__restrict__ src, int tid) {
+    u32x32 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = q * 256 + tid, b = e >> 6;
+        const uint4 v = *reinterpret_cast<const uint4 *>(
+            src + panel_block(b >> 3, (b >> 2) & 1, (b >> 1) & 1, b & 1) * SPLIT_BLOCK + (e & 63) * 8);
+        r[4 * q] = v.x;
+        r[4 * q + 1] = v.y;
+        r[4 * q + 2] = v.z;
+        r[4 * q + 3] = v.w;
+    }
+    return r;
+}
+// ... into the LDS operand layout of mma_split (planes re_hi, re_lo, im_hi,
+// im_lo; row stride SLD)
+__device__ __forceinline__ void put_panel4(uint16_t *dst, const u32x32 r, int tid) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = q * 256 + tid, b = e >> 6, l = e & 63;
+        const int plane = ((b >> 2) & 1) * 2 + ((b >> 1) & 1);
+        const int row = (b & 1) * 32 + (l & 31), k = (b >> 3) * 16 + (l >> 5) * 8;
+        *reinterpret_cast<uint4 *>(dst + plane * SPL + row * SLD + k) =
+            make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    }
+}
+
 template <bool THREE>
 __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ pairs,
@@ -628,7 +675,7 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     const int m = mats[item / npairs], q = (int)(item % npairs);
     const int i = pairs[2 * q], j = pairs[2 * q + 1];
     float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
-    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
+    const uint16_t *sp = split + (int64_t)m * split_stride * PANEL_TILE;
     // steps of the group with both L(i,t) and U(t,j) present (groups <= 32 steps)
     uint32_t mask = (uint32_t)pair_mask[q];
     floatx16 cre, cim;
@@ -636,12 +683,12 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
     if (mask) {
         int t = g0 + __builtin_ctz(mask);
         mask &= mask - 1;
-        u32x32 ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
-        u32x32 rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
+        u32x32 ra = fetch_panel4(sp + (int64_t)ppos[(int64_t)t * nt + i] * PANEL_TILE, tid);
+        u32x32 rb = fetch_panel4(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * PANEL_TILE, tid);
         for (;;) {
             __syncthreads();  // the previous step's products are done with LDS
-            put_split(As, ra, tid);
-            put_split(Bs, rb, tid);
+            put_panel4(As, ra, tid);
+            put_panel4(Bs, rb, tid);
             __syncthreads();
             const bool more = mask != 0;
             if (more) {
@@ -650,8 +697,8 @@ __global__ void __launch_bounds__(256, 2) k_schur_group_split(
             }
             // the next step's tiles (the current step's again after the
             // last), their loads spread over this step's products
-            ra = fetch_split(sp + (int64_t)ppos[(int64_t)t * nt + i] * SPLIT_TILE, tid);
-            rb = fetch_split(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * SPLIT_TILE, tid);
+            ra = fetch_panel4(sp + (int64_t)ppos[(int64_t)t * nt + i] * PANEL_TILE, tid);
+            rb = fetch_panel4(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * PANEL_TILE, tid);
             mma_split<THREE>(As, Bs, cre, cim, wave, lane);
 #pragma unroll
             for (int g = 0; g < 16; ++g) {
@@ -670,56 +717,6 @@ template __global__ void k_schur_group_split<true>(float *, int64_t, int, int, c
 template __global__ void k_schur_group_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *,
                                                     int, int, const int32_t *, const uint16_t *, int64_t,
                                                     const int32_t *, const int32_t *, const int32_t *);
-
-// Trailing update of a step group on 2 x 2 blocks of targets (BF16 precisions):
-// quad q = (i0, i1, j0, j1) (an index -1 is absent); wave w updates the full
-// 64 x 64 target (i_{w/2}, j_{w%2}) over the steps where both of its operands
-// exist, from the split tiles L(i0,t), L(i1,t), U(t,j0), U(t,j1) staged once
-// per step for all four waves -- half the operand traffic per target update
-// of the one-target form.  One block per CU (147 KiB of LDS); the next step's
-// four tiles are loaded into registers while the current products run.
-__device__ __forceinline__ void mma_split_quad_chunk(const uint16_t *As, const uint16_t *Bs, floatx16 (&cre)[4],
-                                                     floatx16 (&cim)[4], int lane, int k0, bool three) {
-    const int kb = (lane >> 5) * 8, r = lane & 31;
-    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int ao = (h * 32 + r) * SLD + k0 + kb;
-        prh[h] = lds8(As + ao);
-        pih[h] = lds8(As + 2 * SPL + ao);
-        urh[h] = lds8(Bs + ao);
-        uih[h] = lds8(Bs + 2 * SPL + ao);
-        if (three) {
-            prl[h] = lds8(As + SPL + ao);
-            pil[h] = lds8(As + 3 * SPL + ao);
-            url[h] = lds8(Bs + SPL + ao);
-            uil[h] = lds8(Bs + 3 * SPL + ao);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const bf16x8 npih = neg8(pih[a]);
-        const bf16x8 npil = three ? neg8(pil[a]) : npih;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            floatx16 &cr = cre[2 * a + b], &ci = cim[2 * a + b];
-            cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], urh[b], cr, 0, 0, 0);
-            cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uih[b], cr, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], uih[b], ci, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih[a], urh[b], ci, 0, 0, 0);
-            if (three) {
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], url[b], cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl[a], urh[b], cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npih, uil[b], cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_bf16(npil, uih[b], cr, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh[a], uil[b], ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl[a], uih[b], ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih[a], url[b], ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pil[a], urh[b], ci, 0, 0, 0);
-            }
-        }
-    }
-}
 
 // Accumulators of a full 64 x 64 tile held by one wave: block 2a+b covers rows
 // 32a.., columns 32b.. with the 32 x 32 MFMA layout.
@@ -750,210 +747,183 @@ __device__ __forceinline__ void store_acc_tile(float *__restrict__ t, const floa
     }
 }
 
+// Trailing Schur updates by the three-multiplication complex product (BF16
+// precisions).  A(i,j) -= sum_t L(i,t) U(t,j) is applied as C += P U with
+// P = -L from the split panel tiles:
+//   T1 = Pr Ur,  T2 = Pi Ui,  T3 = (Pr + Pi)(Ur + Ui),
+//   Re C += T1 - T2,  Im C += T3 - T1 - T2,
+// each real product as hi*hi + hi*lo + lo*hi (THREE) or hi*hi, FP32
+// accumulation.  Three accumulators per 32 x 32 block start at x1 = Re C,
+// x2 = 0, x3 = Re C + Im C and end as Re C = x1 - x2, Im C = x3 - x1 - x2.
+// Accumulator x_c takes, per step and K quarter, its hi*hi, hi*lo and lo*hi
+// products in that order in both kernels below, so they agree bit for bit.
+
+// Reference form, one target per workgroup (wave w: the 32 x 32 block
+// (w / 2, w % 2)): both split tiles of a step are copied whole into LDS
+// (96 KiB, one workgroup per CU).  Runs the trailing pairs of a group over
+// the matrices listed in `mats`.
 template <bool THREE>
-__global__ void __launch_bounds__(256, 1) k_schur_quad_split(
+__global__ void __launch_bounds__(256, 1) k_schur_pairs_3m(
+    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ pairs,
+    const int32_t *__restrict__ pair_mask, int npairs, int nmat, const int32_t *__restrict__ tile_id,
+    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
+    const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[PANEL_TILE], Bs[PANEL_TILE];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t total = (int64_t)npairs * nmat, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int m = mats[item / npairs], q = (int)(item % npairs);
+    const int i = pairs[2 * q], j = pairs[2 * q + 1];
+    float *C = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)i * nt + j]);
+    const uint16_t *sp = split + (int64_t)m * split_stride * PANEL_TILE;
+    const int a = wave >> 1, b = wave & 1;
+    floatx16 x[3];
+    load_acc(C, x[0], x[2], wave, lane);
+    x[2] += x[0];
+    x[1] = floatx16{};
+    for (uint32_t mask = (uint32_t)pair_mask[q]; mask; mask &= mask - 1) {
+        const int t = g0 + __builtin_ctz(mask);
+        const uint4 *la = reinterpret_cast<const uint4 *>(sp + (int64_t)ppos[(int64_t)t * nt + i] * PANEL_TILE);
+        const uint4 *ub = reinterpret_cast<const uint4 *>(sp + (int64_t)(ppos[(int64_t)t * nt + j] + step_s[t]) * PANEL_TILE);
+        __syncthreads();  // the previous step's products are done with LDS
+        for (int e = tid; e < PANEL_TILE / 8; e += 256) {
+            reinterpret_cast<uint4 *>(As)[e] = la[e];
+            reinterpret_cast<uint4 *>(Bs)[e] = ub[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bf16x8 ah = lds8(As + panel_block(kq, c, 0, a) * SPLIT_BLOCK + lane * 8);
+                const bf16x8 bh = lds8(Bs + panel_block(kq, c, 0, b) * SPLIT_BLOCK + lane * 8);
+                x[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, x[c], 0, 0, 0);
+                if (THREE) {
+                    const bf16x8 al = lds8(As + panel_block(kq, c, 1, a) * SPLIT_BLOCK + lane * 8);
+                    const bf16x8 bl = lds8(Bs + panel_block(kq, c, 1, b) * SPLIT_BLOCK + lane * 8);
+                    x[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, x[c], 0, 0, 0);
+                    x[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, x[c], 0, 0, 0);
+                }
+            }
+    }
+    const floatx16 cre = x[0] - x[1], cim = x[2] - x[0] - x[1];
+    store_acc(C, cre, cim, wave, lane);
+}
+
+template __global__ void k_schur_pairs_3m<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                               const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                               const int32_t *, const int32_t *);
+template __global__ void k_schur_pairs_3m<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int,
+                                                int, const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                                const int32_t *, const int32_t *);
+
+// Trailing update of a step group on 2 x 2 blocks of targets (the production
+// form).  Quad q = (i0, i1, j0, j1) (an index -1 is absent); wave w owns the
+// 64 x 64 target (i_{w/2}, j_{w%2}) over the steps where both of its operands
+// exist, with three accumulators per 32 x 32 block (192 registers).  The
+// split tiles L(i0,t), L(i1,t), U(t,j0), U(t,j1) of each listed step t pass
+// through LDS in stages of one K quarter and one component: the hi and lo
+// blocks of both row halves of the four tiles, 16 KiB, of which a wave reads
+// its A and B halves, 8 KiB.  Stages cycle through four buffers (64 KiB, two
+// workgroups per CU: while one waits at a barrier or on its target tiles,
+// the other one's products keep the matrix cores busy).  Each stage opens
+// with one barrier, after which the buffer read in the previous stage is
+// refilled three stages ahead: the barrier certifies that the stage's loads
+// have landed (every wave first waits for its own) and that every wave has
+// read that buffer (its reads fed the previous stage's products).  The loads
+// go from memory straight into LDS (buffer loads with the LDS destination: no
+// registers, no LDS write instructions).  A step's 12 stages (4 quarters x 3
+// components) are unrolled, so every stage's buffer and component are known
+// at compile time.  Waves without an update in a step multiply zeroed A
+// operands instead of branching.
+constexpr int G_STAGE = 16 * SPLIT_BLOCK;  // BF16 elements per stage buffer (16 KiB)
+
+// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0] | expcnt << 4 | lgkmcnt << 8
+// | vmcnt[5:4] << 14): vmcnt <= 8 or <= 4 together with lgkmcnt = 0, and
+// vmcnt = 0 alone
+constexpr int WAIT_VM8_LGKM0 = 0x0078, WAIT_VM4_LGKM0 = 0x0074, WAIT_VM0 = 0x0F70;
+
+// wave w moves the blocks (K quarter q, component c) of its tile -- hi of
+// both row halves, then lo -- from split slot s into stage buffer `buf`
+template <bool THREE>
+__device__ __forceinline__ void g_issue(uint16_t *buf, const uint16_t *__restrict__ sp, int s, int q, int c, int wave,
+                                        int lane) {
+    // buffer (not flat) loads: the compiler then tracks them as memory loads
+    // into LDS only, and the LDS reads of the other buffers wait for none of them
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t *>(sp + (int64_t)s * PANEL_TILE), 0, PANEL_TILE * 2, 0x00020000);
+    uint16_t *d = buf + wave * 4 * SPLIT_BLOCK;
+    const int first = panel_block(q, c, 0, 0);  // blocks first + 2h + a
+#pragma unroll
+    for (int j = 0; j < (THREE ? 4 : 2); ++j)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, reinterpret_cast<void *>(d + j * SPLIT_BLOCK), 16, lane * 16,
+                                                 (first + j) * SPLIT_BLOCK * 2, 0, 0);
+}
+// the split slot of wave w's tile in a listed step's slot entry
+__device__ __forceinline__ int wave_slot(const int4 &slot, int wave) {
+    return __builtin_amdgcn_readfirstlane(wave == 0 ? slot.x : wave == 1 ? slot.y : wave == 2 ? slot.z : slot.w);
+}
+
+__device__ __forceinline__ bf16x8 lds8_and(const uint16_t *p, uint32_t keep) {
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, lds8(p)) & (u32x4)(keep));
+}
+
+// One stage's operands of a wave: A from tile ua (masked by `keep`), B from
+// tile ub; [row half]
+struct StageOps {
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+};
+template <bool THREE>
+__device__ __forceinline__ StageOps g_read(const uint16_t *buf, int ua, int ub, int lane, uint32_t keep) {
+    auto blk = [&](int u, int j) { return buf + (u * 4 + j) * SPLIT_BLOCK + lane * 8; };
+    StageOps o;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        o.ah[h] = lds8_and(blk(ua, h), keep);
+        o.bh[h] = lds8(blk(ub, h));
+        if (THREE) {
+            o.al[h] = lds8_and(blk(ua, 2 + h), keep);
+            o.bl[h] = lds8(blk(ub, 2 + h));
+        }
+    }
+    return o;
+}
+// The stage's products into the component's four accumulators, which take
+// turns so that consecutive MFMAs never depend on each other.
+template <bool THREE>
+__device__ __forceinline__ void g_products(const StageOps &o, floatx16 (&x)[4]) {
+#pragma unroll
+    for (int p = 0; p < (THREE ? 3 : 1); ++p)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int a = t >> 1, b = t & 1;
+            const bf16x8 u = p == 2 ? o.al[a] : o.ah[a];
+            const bf16x8 v = p == 1 ? o.bl[b] : o.bh[b];
+            x[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(u, v, x[t], 0, 0, 0);
+        }
+}
+
+template <bool THREE>
+__global__ void __launch_bounds__(256, 2) k_schur_quad_3m(
     float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
     const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
     const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
     const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
-    __shared__ __attribute__((aligned(16))) uint16_t A0[4 * SPL], A1[4 * SPL], B0[4 * SPL], B1[4 * SPL];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // four buffers as four LDS objects: the compiler then knows that the
+    // reads of one do not wait for the direct loads into another
+    __shared__ __attribute__((aligned(1024))) uint16_t b0[G_STAGE];
+    __shared__ __attribute__((aligned(1024))) uint16_t b1[G_STAGE];
+    __shared__ __attribute__((aligned(1024))) uint16_t b2[G_STAGE];
+    __shared__ __attribute__((aligned(1024))) uint16_t b3[G_STAGE];
+    __shared__ int4 slots[32];  // split slots of the four tiles of the i-th listed step
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
     const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (item >= total) return;
     const int m = mats[item / nquads], q = (int)(item % nquads);
     const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
-    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
-    // step masks: rows (L present), columns (U present), this wave's target
-    const int4 qm = reinterpret_cast<const int4 *>(quad_mask)[q];
-    const uint32_t rm[2] = {(uint32_t)qm.x, (uint32_t)qm.y}, cm[2] = {(uint32_t)qm.z, (uint32_t)qm.w};
-    const int ia = wave >> 1, jb = wave & 1;
-    uint32_t mine = rm[ia] & cm[jb];
-    // steps where some wave has work, and the tiles they need
-    uint32_t all = (rm[0] & cm[0]) | (rm[0] & cm[1]) | (rm[1] & cm[0]) | (rm[1] & cm[1]);
-    const uint32_t need_r[2] = {rm[0] & (cm[0] | cm[1]), rm[1] & (cm[0] | cm[1])};
-    const uint32_t need_c[2] = {cm[0] & (rm[0] | rm[1]), cm[1] & (rm[0] | rm[1])};
-    float *C = mine ? tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)I[ia] * nt + J[jb]]) : nullptr;
-    floatx16 cre[4], cim[4];
-    if (mine) load_acc_tile(C, cre, cim, lane);
-    if (all) {
-        // Offsets of the four split tiles of step t; an absent tile points at
-        // a present one, so that all four loads are issued without branches
-        // (they are not staged).
-        auto offsets = [&](int t, int64_t (&o)[4]) {
-            const uint32_t bit = 1u << (t - g0);
-            const int32_t *pp = ppos + (int64_t)t * nt;
-            const int64_t s0 = (need_r[0] & bit) ? pp[I[0]] : -1, s1 = (need_r[1] & bit) ? pp[I[1]] : -1;
-            const int64_t s2 = (need_c[0] & bit) ? pp[J[0]] + step_s[t] : -1;
-            const int64_t s3 = (need_c[1] & bit) ? pp[J[1]] + step_s[t] : -1;
-            const int64_t any = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : (s2 >= 0 ? s2 : s3));
-            o[0] = (s0 >= 0 ? s0 : any) * SPLIT_TILE;
-            o[1] = (s1 >= 0 ? s1 : any) * SPLIT_TILE;
-            o[2] = (s2 >= 0 ? s2 : any) * SPLIT_TILE;
-            o[3] = (s3 >= 0 ? s3 : any) * SPLIT_TILE;
-        };
-        int t = g0 + __builtin_ctz(all);
-        all &= all - 1;
-        int64_t o[4];
-        offsets(t, o);
-        u32x32 ra0 = fetch_split(sp + o[0], tid), ra1 = fetch_split(sp + o[1], tid);
-        u32x32 rb0 = fetch_split(sp + o[2], tid), rb1 = fetch_split(sp + o[3], tid);
-        for (;;) {
-            const uint32_t bit = 1u << (t - g0);
-            __syncthreads();  // the previous step's products are done with LDS
-            if (need_r[0] & bit) put_split(A0, ra0, tid);
-            if (need_r[1] & bit) put_split(A1, ra1, tid);
-            if (need_c[0] & bit) put_split(B0, rb0, tid);
-            if (need_c[1] & bit) put_split(B1, rb1, tid);
-            __syncthreads();
-            const bool more = all != 0;
-            const int tcur = t;
-            // the next step's tiles (the current step's again after the last)
-            if (more) {
-                t = g0 + __builtin_ctz(all);
-                all &= all - 1;
-            }
-            offsets(t, o);
-            if (mine & (1u << (tcur - g0))) {
-                // the next step's loads spread over this step's products: one
-                // load per few MFMAs, so the products never wait behind a
-                // full memory queue
-                ra0 = fetch_split(sp + o[0], tid);
-                ra1 = fetch_split(sp + o[1], tid);
-                rb0 = fetch_split(sp + o[2], tid);
-                rb1 = fetch_split(sp + o[3], tid);
-                const uint16_t *As = ia ? A1 : A0, *Bs = jb ? B1 : B0;
-#pragma unroll
-                for (int k0 = 0; k0 < TILE; k0 += 16) mma_split_quad_chunk(As, Bs, cre, cim, lane, k0, THREE);
-#pragma unroll
-                for (int g = 0; g < 32; ++g) {
-                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);          // 1 VMEM read
-                    __builtin_amdgcn_sched_group_barrier(0x008, THREE ? 6 : 2, 0);  // MFMA
-                }
-            } else {
-                ra0 = fetch_split(sp + o[0], tid);
-                ra1 = fetch_split(sp + o[1], tid);
-                rb0 = fetch_split(sp + o[2], tid);
-                rb1 = fetch_split(sp + o[3], tid);
-            }
-            if (!more) break;
-        }
-    }
-    if (mine) store_acc_tile(C, cre, cim, lane);
-}
-
-template __global__ void k_schur_quad_split<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                                 const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
-template __global__ void k_schur_quad_split<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                                  const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
-
-
-// Trailing update of a step group on 2 x 2 blocks of targets, pipelined by K
-// quarters (BF16 precisions): the same quads, step masks and products as
-// k_schur_quad_split, each accumulator in the same order, so the results
-// agree bit for bit.  Each step's four split tiles pass through LDS in four
-// quarters of 16 K columns, double-buffered (2 x 32 KiB), which leaves room
-// for two blocks per CU: while one block waits at its barrier or on its
-// target tiles, the other one's products keep the matrix cores busy.  The
-// quarters move from memory straight into LDS (buffer loads with the LDS
-// destination: no registers and no LDS write instructions), one quarter
-// ahead.  Split tiles are stored as operand blocks in lane order, so every
-// block is one 1 KiB load and one conflict-free 16-byte-per-lane LDS read.
-constexpr int QQ_BUF = 4 * 8 * SPLIT_BLOCK;  // four tiles x eight operand blocks (32 KiB)
-
-// wave w moves the 8 operand blocks (plane, row half) of K quarter q of tile w
-// into buffer `buf`
-__device__ __forceinline__ void qq_issue(uint16_t *buf, const uint16_t *__restrict__ sp, int4 slot, int q, int wave,
-                                         int lane) {
-    const int s = __builtin_amdgcn_readfirstlane(wave == 0 ? slot.x : wave == 1 ? slot.y : wave == 2 ? slot.z : slot.w);
-    // buffer (not flat) loads: the compiler then tracks them as memory loads
-    // into LDS only, and the LDS reads of the other buffer wait for none of them
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint16_t *>(sp + (int64_t)s * SPLIT_TILE), 0, SPLIT_TILE * 2, 0x00020000);
-    uint16_t *d = buf + wave * 8 * SPLIT_BLOCK;
-    const int first = (q >> 1) * 16 + (q & 1);  // block of (plane 0, row half 0) in quarter q
-#pragma unroll
-    for (int j = 0; j < 8; ++j)  // j = 2 * plane + row half
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, reinterpret_cast<void *>(d + j * SPLIT_BLOCK), 16,
-                                                 ((first + 2 * j) * SPLIT_BLOCK + lane * 8) * 2, 0, 0, 0);
-}
-
-// products of one quarter buffer: A from tile ua, B from tile ub.  `keep` is
-// all ones, or zero when this wave has no update in the step: its A operands
-// are then zero and the products add nothing (instead of a branch, which would
-// keep the loads and the products of the other waves apart).  Each of the 8
-// accumulators takes its six products in the order of mma_split_quad_chunk;
-// the accumulators take turns, so consecutive MFMAs never depend on each other.
-__device__ __forceinline__ bf16x8 lds8_and(const uint16_t *p, uint32_t keep) {
-    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, lds8(p)) & (u32x4)(keep));
-}
-// The 16 operand blocks of one quarter buffer, read into registers before
-// any product (A from tile ua, masked by `keep`; B from tile ub).
-struct QuarterOps {
-    bf16x8 prh[2], pih[2], prl[2], pil[2], urh[2], uih[2], url[2], uil[2];
-};
-__device__ __forceinline__ QuarterOps qq_read(const uint16_t *buf, int ua, int ub, int lane, bool three,
-                                              uint32_t keep) {
-    auto blk = [&](int u, int pl, int a) { return buf + (u * 8 + pl * 2 + a) * SPLIT_BLOCK + lane * 8; };
-    QuarterOps o;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        o.prh[h] = lds8_and(blk(ua, 0, h), keep);
-        o.pih[h] = lds8_and(blk(ua, 2, h), keep);
-        o.urh[h] = lds8(blk(ub, 0, h));
-        o.uih[h] = lds8(blk(ub, 2, h));
-        if (three) {
-            o.prl[h] = lds8_and(blk(ua, 1, h), keep);
-            o.pil[h] = lds8_and(blk(ua, 3, h), keep);
-            o.url[h] = lds8(blk(ub, 1, h));
-            o.uil[h] = lds8(blk(ub, 3, h));
-        }
-    }
-    return o;
-}
-// The products of one quarter.  Each of the 8 accumulators takes its six
-// products in the order of mma_split_quad_chunk; the accumulators take turns,
-// so consecutive MFMAs never depend on each other.
-__device__ __forceinline__ void qq_products(const QuarterOps &o, floatx16 (&cre)[4], floatx16 (&cim)[4],
-                                            bool three) {
-    bf16x8 npih[2], npil[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        npih[h] = neg8(o.pih[h]);
-        npil[h] = three ? neg8(o.pil[h]) : npih[h];
-    }
-    const int np = three ? 6 : 2;
-#pragma unroll
-    for (int p = 0; p < np; ++p)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int a = t >> 1, b = t & 1;
-            // re += Pr Ur - Pi Ui (hi hi, then hi lo, lo hi), im += Pr Ui + Pi Ur
-            const bf16x8 xr[6] = {o.prh[a], npih[a], o.prh[a], o.prl[a], npih[a], npil[a]};
-            const bf16x8 yr[6] = {o.urh[b], o.uih[b], o.url[b], o.urh[b], o.uil[b], o.uih[b]};
-            const bf16x8 xi[6] = {o.prh[a], o.pih[a], o.prh[a], o.prl[a], o.pih[a], o.pil[a]};
-            const bf16x8 yi[6] = {o.uih[b], o.urh[b], o.uil[b], o.uih[b], o.url[b], o.urh[b]};
-            cre[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xr[p], yr[p], cre[t], 0, 0, 0);
-            cim[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xi[p], yi[p], cim[t], 0, 0, 0);
-        }
-}
-
-// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0] | expcnt << 4 | lgkmcnt << 8
-// | vmcnt[5:4] << 14) that wait for one counter only
-constexpr int WAIT_VM8 = 0x0F78, WAIT_VM0 = 0x0F70, WAIT_LGKM0 = 0xC07F;
-
-// One quad (matrix m, quad q) of the trailing update, K quarters double-
-// buffered through lds0/lds1 (see k_schur_quad_pp).  Called by every wave of
-// the workgroup; leaves both buffers idle and slots free on return.
-template <bool THREE>
-__device__ __forceinline__ void quad_pp_body(int m, int q, float *__restrict__ tiles, int64_t mat_stride, int nt, int g0,
-                                             const int32_t *__restrict__ quads, const int32_t *__restrict__ quad_mask,
-                                             const int32_t *__restrict__ tile_id, const uint16_t *__restrict__ split,
-                                             int64_t split_stride, const int32_t *__restrict__ ppos,
-                                             const int32_t *__restrict__ step_s, uint16_t *lds0, uint16_t *lds1,
-                                             int4 *slots, int tid, int wave, int lane) {
-    const int I[2] = {quads[4 * q], quads[4 * q + 1]}, J[2] = {quads[4 * q + 2], quads[4 * q + 3]};
-    const uint16_t *sp = split + (int64_t)m * split_stride * SPLIT_TILE;
+    const uint16_t *sp = split + (int64_t)m * split_stride * PANEL_TILE;
     const int4 qm = reinterpret_cast<const int4 *>(quad_mask)[q];
     const uint32_t rm[2] = {(uint32_t)qm.x, (uint32_t)qm.y}, cm[2] = {(uint32_t)qm.z, (uint32_t)qm.w};
     const int ia = wave >> 1, jb = wave & 1;
@@ -966,95 +936,96 @@ __device__ __forceinline__ void quad_pp_body(int m, int q, float *__restrict__ t
     {
         uint32_t rest = all;
         for (int i = 0; rest; ++i) {
-            const int b = __builtin_ctz(rest);
+            const int bt = __builtin_ctz(rest);
             rest &= rest - 1;
-            if (mine & (1u << b)) work |= 1u << i;
+            if (mine & (1u << bt)) work |= 1u << i;
             if (tid == i) {
-                // an absent tile takes a present one's slot: its half is
+                // an absent tile takes a present one's slot: its blocks are
                 // loaded but read by no wave
-                const uint32_t bit = 1u << b;
-                const int32_t *pp = ppos + (int64_t)(g0 + b) * nt;
-                const int ss = step_s[g0 + b];
+                const uint32_t bit = 1u << bt;
+                const int32_t *pp = ppos + (int64_t)(g0 + bt) * nt;
+                const int ss = step_s[g0 + bt];
                 const bool any_c = (cm[0] | cm[1]) & bit, any_r = (rm[0] | rm[1]) & bit;
-                int s0 = (rm[0] & bit) && any_c ? pp[I[0]] : -1, s1 = (rm[1] & bit) && any_c ? pp[I[1]] : -1;
-                int s2 = (cm[0] & bit) && any_r ? pp[J[0]] + ss : -1, s3 = (cm[1] & bit) && any_r ? pp[J[1]] + ss : -1;
+                const int s0 = (rm[0] & bit) && any_c ? pp[I[0]] : -1, s1 = (rm[1] & bit) && any_c ? pp[I[1]] : -1;
+                const int s2 = (cm[0] & bit) && any_r ? pp[J[0]] + ss : -1;
+                const int s3 = (cm[1] & bit) && any_r ? pp[J[1]] + ss : -1;
                 const int any = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : (s2 >= 0 ? s2 : s3));
                 slots[i] = make_int4(s0 >= 0 ? s0 : any, s1 >= 0 ? s1 : any, s2 >= 0 ? s2 : any, s3 >= 0 ? s3 : any);
             }
         }
     }
     float *C = mine ? tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)I[ia] * nt + J[jb]]) : nullptr;
-    floatx16 cre[4], cim[4];
+    const int ua = ia, ub = 2 + jb;
+    uint16_t *const bufs[4] = {b0, b1, b2, b3};
+    // stage u (0..11) of a listed step: K quarter u / 3, component u % 3,
+    // buffer u % 4 (a step has 12 stages); past the end the last step's
+    // stages are loaded again, into buffers no later stage reads
+    auto issue = [&](int s, int u) { g_issue<THREE>(bufs[u & 3], sp, s, u / 3, u % 3, wave, lane); };
+    __syncthreads();  // the slots are visible
+    // the first three stages load while the target tile does
+    if (n) {
+        const int s0 = wave_slot(slots[0], wave);
+        issue(s0, 0);
+        issue(s0, 1);
+        issue(s0, 2);
+    }
+    floatx16 x[3][4];
     if (mine) {
-        load_acc_tile(C, cre, cim, lane);
+        load_acc_tile(C, x[0], x[2], lane);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            x[2][t] += x[0][t];
+            x[1][t] = floatx16{};
+        }
     } else {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) cre[b] = cim[b] = floatx16{};
+        for (int t = 0; t < 4; ++t) x[0][t] = x[1][t] = x[2][t] = floatx16{};
     }
-    __syncthreads();
     if (n) {
-        const int ua = ia, ub = 2 + jb;
-        const int nq = 4 * n;  // K quarters of the listed steps, flattened
-        // quarter g (step g / 4, K quarter g % 4) goes to buffer g % 2; past
-        // the end the last quarter is loaded again, into a buffer no later
-        // quarter reads
-        auto issue = [&](uint16_t *buf, int g) {
-            const int gg = g < nq ? g : nq - 1;
-            qq_issue(buf, sp, slots[gg >> 2], gg & 3, wave, lane);
-        };
-        issue(lds0, 0);
-        issue(lds1, 1);
-        // One quarter: wait for its loads (the next quarter's stay in flight),
-        // barrier (its four tiles are in LDS for every wave), read this wave's
-        // operands into registers, barrier (every wave has read: the buffer is
-        // free), refill the buffer two quarters ahead, multiply.  The loads
-        // thus have two quarters of products to land.  The barriers are the
-        // bare instruction: a __syncthreads() fence would drain the loads in
-        // flight.
-        auto quarter = [&](uint16_t *buf, int g) {
-            __builtin_amdgcn_s_waitcnt(WAIT_VM8);
-            __builtin_amdgcn_s_barrier();
-            const uint32_t keep = (work >> (g >> 2)) & 1u ? 0xffffffffu : 0u;
-            const QuarterOps ops = qq_read(buf, ua, ub, lane, THREE, keep);
-            __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
-            __builtin_amdgcn_s_barrier();
-            issue(buf, g + 2);
-            __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products
-            qq_products(ops, cre, cim, THREE);
-        };
-        for (int g = 0; g < nq; g += 2) {
-            quarter(lds0, g);
-            quarter(lds1, g + 1);
+        for (int i = 0; i < n; ++i) {
+            const uint32_t keep = (work >> i) & 1u ? 0xffffffffu : 0u;
+            // this step's and the next step's slots, looked up once
+            const int sc = wave_slot(slots[i], wave), sn = wave_slot(slots[i + 1 < n ? i + 1 : n - 1], wave);
+#pragma unroll
+            for (int u = 0; u < 12; ++u) {
+                // this stage's loads land (the next two stages' stay in
+                // flight) and this wave's reads of the previous stage are
+                // done; the barrier then makes every wave's loads visible and
+                // frees the previous stage's buffer for a refill
+                __builtin_amdgcn_s_waitcnt(THREE ? WAIT_VM8_LGKM0 : WAIT_VM4_LGKM0);
+                __builtin_amdgcn_s_barrier();
+                issue(u + 3 < 12 ? sc : sn, (u + 3) % 12);
+                const StageOps o = g_read<THREE>(bufs[u & 3], ua, ub, lane, keep);
+                __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products
+                // the products run at raised wave priority, so that the other
+                // wave on the SIMD takes its barrier and reads meanwhile
+                __builtin_amdgcn_s_setprio(1);
+                g_products<THREE>(o, x[u % 3]);
+                __builtin_amdgcn_s_setprio(0);
+                __builtin_amdgcn_sched_barrier(0);  // ... which stay ahead of the next barrier
+            }
         }
         // the loads issued past the end land before LDS is reused
         __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         __syncthreads();
     }
-    if (mine) store_acc_tile(C, cre, cim, lane);
+    if (mine) {
+        floatx16 cre[4], cim[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            cre[t] = x[0][t] - x[1][t];
+            cim[t] = x[2][t] - x[0][t] - x[1][t];
+        }
+        store_acc_tile(C, cre, cim, lane);
+    }
 }
 
-template <bool THREE>
-__global__ void __launch_bounds__(256, 2) k_schur_quad_pp(
-    float *__restrict__ tiles, int64_t mat_stride, int nt, int g0, const int32_t *__restrict__ quads,
-    const int32_t *__restrict__ quad_mask, int nquads, int nmat, const int32_t *__restrict__ tile_id,
-    const uint16_t *__restrict__ split, int64_t split_stride, const int32_t *__restrict__ ppos,
-    const int32_t *__restrict__ step_s, const int32_t *__restrict__ mats) {
-    // two buffers as two LDS objects: the compiler then knows that the reads
-    // of one do not wait for the direct loads into the other
-    __shared__ __attribute__((aligned(1024))) uint16_t lds0[QQ_BUF];
-    __shared__ __attribute__((aligned(1024))) uint16_t lds1[QQ_BUF];
-    __shared__ int4 slots[32];  // split slots of the four tiles of the i-th listed step
-    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int64_t total = (int64_t)nquads * nmat, per = (total + 7) / 8;
-    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (item >= total) return;
-    quad_pp_body<THREE>(mats[item / nquads], (int)(item % nquads), tiles, mat_stride, nt, g0, quads, quad_mask,
-                        tile_id, split, split_stride, ppos, step_s, lds0, lds1, slots, tid, wave, lane);
-}
+template __global__ void k_schur_quad_3m<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
+                                              const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                              const int32_t *, const int32_t *);
+template __global__ void k_schur_quad_3m<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int,
+                                               int, const int32_t *, const uint16_t *, int64_t, const int32_t *,
+                                               const int32_t *, const int32_t *);
 
-template __global__ void k_schur_quad_pp<true>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                              const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
-template __global__ void k_schur_quad_pp<false>(float *, int64_t, int, int, const int32_t *, const int32_t *, int, int,
-                                               const int32_t *, const uint16_t *, int64_t, const int32_t *, const int32_t *, const int32_t *);
 
 }  // namespace aosx

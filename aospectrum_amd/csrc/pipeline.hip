@@ -112,7 +112,16 @@ void Bundle::flush_ranges(std::vector<TimedRange> &rs, bool destroy) {
     if (destroy) rs.clear();
 }
 
+// Device allocation and release synchronise the device, and a stream capture
+// running in another thread meanwhile is invalidated by HIP (a serving
+// process runs one bundle per thread).  Captures and every allocation,
+// release and graph destruction of all bundles in the process therefore take
+// this lock; it is recursive because a capturing thread may re-enter it.
+static std::recursive_mutex g_device_lock;
+using DeviceLock = std::lock_guard<std::recursive_mutex>;
+
 void Bundle::clear_graphs() {
+    DeviceLock lock(g_device_lock);
     for (auto &kv : graphs) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         for (TimedRange &r : kv.second.ranges) {
@@ -147,6 +156,7 @@ void Bundle::run_sequence(const std::vector<int64_t> &key_in, F &&body) {
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         if (graphs.size() >= 512) clear_graphs();
+        DeviceLock lock(g_device_lock);  // no allocation or release anywhere during the capture
         GraphEntry e;
         ranges = &e.ranges;
         capturing = true;
@@ -203,6 +213,7 @@ static void *scratch_address(int device) {
 
 Bundle::~Bundle() {
     if (device >= 0) {
+        DeviceLock lock(g_device_lock);
         (void)hipSetDevice(device);
         clear_graphs();
         release_workspace();
@@ -220,6 +231,7 @@ Bundle::~Bundle() {
 void *Bundle::raw(const std::string &name, size_t bytes) {
     Slot &s = ws[name];
     if (s.bytes >= bytes && s.ptr) return s.ptr;
+    DeviceLock lock(g_device_lock);
     if (s.ptr) {
         AOSX_HIP(hipStreamSynchronize(stream));
         AOSX_HIP(hipFree(s.ptr));
@@ -244,6 +256,7 @@ void *Bundle::raw(const std::string &name, size_t bytes) {
 }
 
 void Bundle::release_workspace() {
+    DeviceLock lock(g_device_lock);
     clear_graphs();
     for (auto &kv : ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
@@ -264,6 +277,7 @@ void Bundle::zero(void *p, size_t bytes) {
 
 template <typename T>
 static T *upload_vec(const std::vector<T> &v, uint64_t &counter) {
+    DeviceLock lock(g_device_lock);
     T *p = nullptr;
     const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
     AOSX_HIP(hipMalloc(&p, bytes));
@@ -497,6 +511,7 @@ void Bundle::plan_schur(int group) {
         bwd_row_ptr.push_back((int64_t)brows.size());
     }
     // the launch sequences captured so far follow the old plan
+    DeviceLock lock(g_device_lock);
     clear_graphs();
     for (int32_t *p : {d_pairs, d_quads, d_ppos, d_step_s, d_fwd_rows, d_bwd_rows, d_pair_mask, d_quad_mask})
         if (p) AOSX_HIP(hipFree(p));
@@ -636,7 +651,7 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
     float2 *d_z = buf<float2>("f_z", nmat);
     const bool split_mode = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR ||
                             precision == AOSX_PRECISION_SHIFT_IR;
-    uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * SPLIT_TILE) : nullptr;
+    uint16_t *split = split_mode ? buf<uint16_t>("split", (size_t)nmat * split_tiles * PANEL_TILE) : nullptr;
     AOSX_HIP(hipMemcpyAsync(d_kidx, kidx.data(), nmat * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     AOSX_HIP(hipMemcpyAsync(d_z, z.data(), nmat * sizeof(float2), hipMemcpyHostToDevice, stream));
     // BF16 updates: the matrices taking three products (BF16x3) first, then
@@ -728,16 +743,26 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
             if (np == 0) continue;
             const int r = range_begin(RANGE_SCHUR);
             if (split_mode) {
+                // three-multiplication products: 2 x 2 target quads, or the
+                // one-target reference form (AOSX_QUAD_PIPELINE=0)
                 const int64_t nq = grp_quad_ptr[g + 1] - grp_quad_ptr[g];
                 const int32_t *qd = d_quads + 4 * grp_quad_ptr[g], *qm = d_quad_mask + 4 * grp_quad_ptr[g];
+                const int32_t *pr = d_pairs + 2 * grp_pair_ptr[g], *pm = d_pair_mask + grp_pair_ptr[g];
                 for (const bool three : {true, false}) {
                     const int cnt = three ? n3 : n1;
                     if (cnt == 0) continue;
-                    auto quad = quad_pipeline == 1 ? (three ? k_schur_quad_pp<true> : k_schur_quad_pp<false>)
-                                                   : (three ? k_schur_quad_split<true> : k_schur_quad_split<false>);
-                    quad<<<(unsigned)(8 * ((nq * cnt + 7) / 8)), 256, 0, stream>>>(
-                        tiles, mstride, nt, g0, qd, qm, (int)nq, cnt, d_tile_id, split, split_tiles, d_ppos, d_step_s,
-                        three ? d_mats : d_mats + n3);
+                    const int32_t *mats = three ? d_mats : d_mats + n3;
+                    if (quad_pipeline == 1) {
+                        auto quad = three ? k_schur_quad_3m<true> : k_schur_quad_3m<false>;
+                        quad<<<(unsigned)(8 * ((nq * cnt + 7) / 8)), 256, 0, stream>>>(
+                            tiles, mstride, nt, g0, qd, qm, (int)nq, cnt, d_tile_id, split, split_tiles, d_ppos,
+                            d_step_s, mats);
+                    } else {
+                        auto pairs = three ? k_schur_pairs_3m<true> : k_schur_pairs_3m<false>;
+                        pairs<<<(unsigned)(8 * ((np * cnt + 7) / 8)), 256, 0, stream>>>(
+                            tiles, mstride, nt, g0, pr, pm, (int)np, cnt, d_tile_id, split, split_tiles, d_ppos,
+                            d_step_s, mats);
+                    }
                     AOSX_LAUNCHED();
                 }
             } else {

@@ -185,8 +185,9 @@ class BandServer:
         vectors = n_k * n * width * 8 * 12 + slots * n * width * 8 * 3
         matrices = n_k * int(info["nnz"]) * 16
         # BF16 precisions: split copies of one step group's panel tiles per
-        # factorisation (at most 2 x 32 steps x n_diag tiles of 32 KiB)
-        split = 2 * 32 * int(info["n_diagonal_tiles"]) * 32768 if settings.precision.startswith("bf16") else 0
+        # factorisation (at most 2 x 32 steps x n_diag tiles of 48 KiB)
+        split_mode = settings.precision in ("bf16x3_ir", "bf16_ir", "per_shift_ir")
+        split = 2 * 32 * int(info["n_diagonal_tiles"]) * 49152 if split_mode else 0
         return int(slots * (int(info["factor_bytes"]) + split) + vectors + matrices)
 
     # -------------------------------------------------------------- jobs

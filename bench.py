@@ -49,8 +49,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_ir": 157.3, "bf16x3_ir": 2500.0, "bf16_ir": 2500.0, "per_shift_ir": 2500.0}
-# BF16 split products issued per algorithmic product (hi*hi + hi*lo + lo*hi)
-MFMA_PRODUCTS = {"fp32": 1, "fp32_ir": 1, "bf16x3_ir": 3, "bf16_ir": 1, "per_shift_ir": None}
+# MFMA flops the trailing update issues per algorithmic flop (8 per complex
+# multiply-add): the BF16 forms take three real products per complex product
+# (re*re, im*im, (re+im)*(re+im)), each as hi*hi + hi*lo + lo*hi (BF16x3:
+# 9 x 2 / 8) or hi*hi (3 x 2 / 8); FP32 takes the four textbook products
+MFMA_PRODUCTS = {"fp32": 1, "fp32_ir": 1, "bf16x3_ir": 2.25, "bf16_ir": 0.75, "per_shift_ir": None}
 
 
 def log(*parts):
@@ -63,7 +66,7 @@ def schur_kernel(precision: str) -> str:
     if not (precision.startswith("bf16") or precision == "per_shift_ir"):
         return "k_schur_group"
     three = {"bf16x3_ir": "true", "bf16_ir": "false", "per_shift_ir": "true|false"}[precision]
-    form = "k_schur_quad_split" if os.environ.get("AOSX_QUAD_PIPELINE", "1") == "0" else "k_schur_quad_pp"
+    form = "k_schur_pairs_3m" if os.environ.get("AOSX_QUAD_PIPELINE", "1") == "0" else "k_schur_quad_3m"
     return f"{form}<{three}>"
 
 

@@ -232,9 +232,10 @@ def _bundle_with_env(bundle, **env):
 
 @pytest.mark.parametrize("precision,form", [("bf16x3_ir", 1), ("bf16_ir", 1)])
 def test_pipelined_quad_updates_match_step_kernel(small2d, precision, form):
-    """The trailing quad update pipelined by K quarters accumulates the same
-    products in the same order as the step-at-a-time kernel: with step groups
-    of 4 (three trailing launches per factorisation) both give the same
+    """The trailing update on 2 x 2 quads, staged through LDS by K quarter and
+    component, accumulates the same three-multiplication products in the same
+    order as the one-target reference kernel (AOSX_QUAD_PIPELINE=0): with step
+    groups of 4 (three trailing launches per factorisation) both give the same
     energies and residuals."""
     kpts = np.array([[0.13, 0.37, 0.0], [0.25, 0.1, 0.0]])
     interval = EnergyInterval(-2.02, 2.02)
