@@ -210,12 +210,14 @@ struct Bundle {
     void factor(int nmat, const std::vector<int32_t> &kidx, const std::vector<float2> &z, int precision,
                 float *tiles, float *dinv, int *d_replaced, const std::vector<uint8_t> *one_product = nullptr);
     // precision selects the MFMA form of the solve products (BF16 split for
-    // the BF16 precisions, FP32 otherwise)
-    void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans, int precision);
+    // the BF16 precisions, FP32 otherwise).  pairs = F > 0: nmat = 2F sets,
+    // set m >= F solved with the conjugate transpose of factorisation m - F
+    // (ctrans ignored); pairs = 0: set m with factorisation m and ctrans.
+    void solve(int nmat, int W, float2 *y, float *tiles, float *dinv, bool ctrans, int precision, int pairs = 0);
     void solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
                     const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                     const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
-                    float2 *x_out, double *rel_residual, int precision);
+                    float2 *x_out, double *rel_residual, int precision, int pairs = 0);
     // norm[k][j] = ||H_k v_kj - e_kj S_k v_kj||_2 (FP64) for count pairs per k;
     // H_k, S_k assembled at kpts or given explicitly (host arrays)
     void eig_residual(int64_t nk, const double *kpts, const float2 *h_in, const float2 *s_in, int count,

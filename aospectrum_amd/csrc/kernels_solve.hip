@@ -19,6 +19,20 @@ __device__ __forceinline__ int64_t rhs_block(int64_t mat_stride, int m, int tile
     return (int64_t)m * mat_stride + ((int64_t)tile * nchunk + c) * TILE_FLOATS;
 }
 
+// Right-hand-side sets of a solve launch.  With pairs = F > 0 the launch
+// covers 2F sets: set m < F is solved with factorisation m, set m >= F with
+// the conjugate transpose of factorisation m - F (the conjugate shift of a
+// contour pair, A^H = U^H L^H), so both shifts of a pair take one sweep.  With
+// pairs = 0 set m uses factorisation m and the launch's ctrans.  `which`
+// arguments name the diagonal inverse of the plain orientation (0: inv(L_kk),
+// 1: inv(U_kk)); a conjugate set takes the other one.
+struct SolveSet {
+    int fac, ct;
+};
+__device__ __forceinline__ SolveSet solve_set(int m, int pairs, int ctrans) {
+    return pairs > 0 ? SolveSet{m % pairs, m >= pairs ? 1 : 0} : SolveSet{m, ctrans};
+}
+
 // Stage the conjugate transpose of a stored tile as the A operand.
 __device__ __forceinline__ void stage_a_ctrans(const float *__restrict__ g, float *Are, float *Aim, int tid) {
     for (int e = tid; e < TILE2; e += 256) {
@@ -32,11 +46,13 @@ __device__ __forceinline__ void stage_a_ctrans(const float *__restrict__ g, floa
 // (which 1), conjugate-transposed when ctrans.  FP32 MFMA.
 __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                    const float *__restrict__ dinv, int nt, int k, int which,
-                                                   int ctrans) {
+                                                   int ctrans, int pairs) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int m = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
+    const float *d = dinv + ((int64_t)st.fac * nt + k) * (2 * TILE_FLOATS) + (which ^ st.ct) * TILE_FLOATS;
     if (ctrans) stage_a_ctrans(d, Are, Aim, tid);
     else stage_a(d, Are, Aim, tid);
     for (int c = 0; c < nchunk; ++c) {
@@ -93,10 +109,12 @@ __device__ __forceinline__ void store_acc_split_t(uint16_t *__restrict__ dst, co
 
 __global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                             const float *__restrict__ dinv, int nt, int k, int which,
-                                                            int ctrans, uint16_t *__restrict__ ysplit) {
+                                                            int ctrans, int pairs, uint16_t *__restrict__ ysplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int m = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
+    const float *d = dinv + ((int64_t)st.fac * nt + k) * (2 * TILE_FLOATS) + (which ^ st.ct) * TILE_FLOATS;
     stage_split(d, As, ctrans != 0, false, ctrans != 0, tid);
     for (int c = 0; c < nchunk; ++c) {
         float *yk = y + rhs_block(y_mat_stride, m, k, nchunk, c);
@@ -115,13 +133,15 @@ __global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__
 __global__ void __launch_bounds__(256) k_solve_update(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                      const float *__restrict__ tiles, int64_t mat_stride,
                                                      const int32_t *__restrict__ tile_id, int nt, int k,
-                                                     const int32_t *__restrict__ list, int ctrans) {
+                                                     const int32_t *__restrict__ list, int ctrans, int pairs) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
     const int i = list[blockIdx.x];
     const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
-    const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+    const float *t = tiles + (int64_t)st.fac * mat_stride + (int64_t)id * TILE_FLOATS;
     if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
     else stage_a(t, Are, Aim, tid);
     for (int c = 0; c < nchunk; ++c) {
@@ -143,12 +163,15 @@ __global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict
                                                               int nchunk, const float *__restrict__ tiles,
                                                               int64_t mat_stride, const int32_t *__restrict__ tile_id,
                                                               int nt, int k, const int32_t *__restrict__ list,
-                                                              int ctrans, const uint16_t *__restrict__ ysplit) {
+                                                              int ctrans, int pairs,
+                                                              const uint16_t *__restrict__ ysplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL];
     const int m = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
     const int i = list[blockIdx.x];
     const int32_t id = ctrans ? tile_id[(int64_t)k * nt + i] : tile_id[(int64_t)i * nt + k];
-    const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+    const float *t = tiles + (int64_t)st.fac * mat_stride + (int64_t)id * TILE_FLOATS;
     stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
     for (int c = 0; c < nchunk; ++c) {
         float *yi = y + rhs_block(y_mat_stride, m, i, nchunk, c);
@@ -177,7 +200,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
                                                        const float *__restrict__ tiles, int64_t mat_stride,
                                                        const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
                                                        const int32_t *__restrict__ rows, int nrows, int nmat,
-                                                       int ctrans, const uint16_t *__restrict__ ysplit) {
+                                                       int ctrans, int pairs, const uint16_t *__restrict__ ysplit) {
     // FP32 operands (Are/Aim/Bre/Bim) or split operands (As/Bs) share one
     // LDS area.
     __shared__ __attribute__((aligned(16))) float lds[SPLIT ? (8 * SPL) / 2 : 2 * TILE * LDA + 2 * TILE2];
@@ -191,6 +214,8 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     const int r = (int)(item % nrows);
     const int64_t mc = item / nrows;
     const int m = (int)(mc / ngrp), c0 = (int)(mc % ngrp) * NC;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
     const int i = rows[r];
     const int nc = min(NC, nchunk - c0);
     // the row's tiles of the group's steps (at most 32), looked up at once:
@@ -213,7 +238,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     if (SPLIT) {
         // the next present tile and its first RHS block are loaded into
         // registers while the current products run
-        const float *tm = tiles + (int64_t)m * mat_stride;
+        const float *tm = tiles + (int64_t)st.fac * mat_stride;
         int k = next_k(g0);
         TileBlock tn;
         u32x32 yn;
@@ -252,7 +277,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     }
     for (int k = next_k(g0); !SPLIT && k < g1; k = next_k(k + 1)) {
         const int32_t id = tile_of(k);
-        const float *t = tiles + (int64_t)m * mat_stride + (int64_t)id * TILE_FLOATS;
+        const float *t = tiles + (int64_t)st.fac * mat_stride + (int64_t)id * TILE_FLOATS;
         __syncthreads();  // the previous products are done with LDS
         if (ctrans) stage_a_ctrans(t, Are, Aim, tid);
         else stage_a(t, Are, Aim, tid);
@@ -285,12 +310,15 @@ __global__ void __launch_bounds__(256) k_solve_group(float *__restrict__ y, int6
                                                      const float *__restrict__ tiles, int64_t mat_stride,
                                                      const int32_t *__restrict__ tile_id, int nt,
                                                      const float *__restrict__ dinv, int g0, int g1, int fwd,
-                                                     int which, int ctrans) {
+                                                     int which, int ctrans, int pairs) {
     __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
     __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
-    const float *tm = tiles + (int64_t)m * mat_stride;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
+    which ^= st.ct;
+    const float *tm = tiles + (int64_t)st.fac * mat_stride;
     const int G = g1 - g0;
     auto tile_of = [&](int k, int t) { return ctrans ? tile_id[(int64_t)t * nt + k] : tile_id[(int64_t)k * nt + t]; };
     for (int s = 0; s < G; ++s) {
@@ -313,7 +341,7 @@ __global__ void __launch_bounds__(256) k_solve_group(float *__restrict__ y, int6
         // D_k times the updated Y_k, read back from memory as the B operand
         store_acc(yk, cre, cim, wave, lane);
         __syncthreads();
-        const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+        const float *d = dinv + ((int64_t)st.fac * nt + k) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
         if (ctrans) stage_a_ctrans(d, Are, Aim, tid);
         else stage_a(d, Are, Aim, tid);
         stage_b(yk, Bre, Bim, tid);
@@ -366,14 +394,17 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
                                                               int nchunk, const float *__restrict__ tiles,
                                                               int64_t mat_stride, const int32_t *__restrict__ tile_id,
                                                               int nt, const float *__restrict__ dinv, int g0, int g1,
-                                                              int fwd, int which, int ctrans,
+                                                              int fwd, int which, int ctrans, int pairs,
                                                               uint16_t *__restrict__ ysplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[4 * SPL], Bs[4 * SPL], Ys[4 * SPL];
     __shared__ int32_t op_id[32 * 32];  // tile of update (s, u), u < s; -1 when absent
     __shared__ uint32_t op_mask[32];    // bit u of step s: update (s, u) present
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
-    const float *tm = tiles + (int64_t)m * mat_stride;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    ctrans = st.ct;
+    which ^= st.ct;
+    const float *tm = tiles + (int64_t)st.fac * mat_stride;
     const int G = g1 - g0;  // at most 32
     auto step_of = [&](int s) { return fwd ? g0 + s : g1 - 1 - s; };
     // the group's tile lookups, all at once: wave w takes steps w, w + 4, ..
@@ -397,7 +428,7 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
         u = rest ? __builtin_ctz(rest) : s;
     };
     auto a_tile = [&](int s, int u) -> const float * {
-        if (u == s) return dinv + ((int64_t)m * nt + step_of(s)) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
+        if (u == s) return dinv + ((int64_t)st.fac * nt + step_of(s)) * (2 * TILE_FLOATS) + which * TILE_FLOATS;
         return tm + (int64_t)op_id[s * 32 + u] * TILE_FLOATS;
     };
     int s = 0, u = 0;  // step 0 has no updates: its diagonal comes first
@@ -446,9 +477,9 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
 }
 
 template __global__ void k_solve_trail<false, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                 int, int, const int32_t *, int, int, int, const uint16_t *);
+                                                 int, int, const int32_t *, int, int, int, int, const uint16_t *);
 template __global__ void k_solve_trail<true, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                int, int, const int32_t *, int, int, int, const uint16_t *);
+                                                int, int, const int32_t *, int, int, int, int, const uint16_t *);
 
 // Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
 // (row-major n x w_src complex64); columns >= w_src and padded rows are zero.

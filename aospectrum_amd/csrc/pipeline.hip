@@ -778,7 +778,7 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
 
 // ------------------------------------------------------------------- solves
 // y: planar tiled RHS (Nt x nchunk blocks per matrix), solved in place.
-void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision) {
+void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool ctrans, int precision, int pairs) {
     const bool split = precision == AOSX_PRECISION_BF16X3_IR || precision == AOSX_PRECISION_BF16_IR ||
                        precision == AOSX_PRECISION_SHIFT_IR;
     float *y = reinterpret_cast<float *>(y2);
@@ -788,23 +788,24 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
     auto trail = split ? k_solve_trail<true, 2> : k_solve_trail<false, 2>;
     // split RHS slots (BF16 precisions): finished Y_k blocks, split once
     uint16_t *ysplit = split ? buf<uint16_t>("s_ysplit", (size_t)nmat * YSLOTS * nchunk * SPLIT_TILE) : nullptr;
-    // Y_k <- D_k Y_k (which: 0 inv(L_kk), 1 inv(U_kk)), and the split copy
+    // Y_k <- D_k Y_k (which: 0 inv(L_kk), 1 inv(U_kk) for a plain set; a
+    // conjugate set takes the other), and the split copy
+    const int ct = ctrans ? 1 : 0;
     auto diag_launch = [&](int k, int which) {
         if (split)
-            k_solve_diag_split<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ctrans ? 1 : 0,
-                                                         ysplit);
+            k_solve_diag_split<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ct, pairs, ysplit);
         else
-            k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ctrans ? 1 : 0);
+            k_solve_diag<<<nmat, 256, 0, stream>>>(y, ystride, nchunk, dinv, nt, k, which, ct, pairs);
         AOSX_LAUNCHED();
     };
     // Y_i -= T_ik Y_k for the `count` rows listed at `rows_dev`
     auto update_launch = [&](int k, const int32_t *rows_dev, int count) {
         if (split)
             k_solve_update_split<<<dim3(count, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
-                                                                         nt, k, rows_dev, ctrans ? 1 : 0, ysplit);
+                                                                         nt, k, rows_dev, ct, pairs, ysplit);
         else
             k_solve_update<<<dim3(count, nmat), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt,
-                                                                   k, rows_dev, ctrans ? 1 : 0);
+                                                                   k, rows_dev, ct, pairs);
         AOSX_LAUNCHED();
     };
     const int ngrp = (nchunk + 1) / 2;
@@ -814,18 +815,18 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         const unsigned blocks = (unsigned)(nmat * nchunk);
         if (split)
             k_solve_group_split<<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, dinv,
-                                                            g0, g1, fwd ? 1 : 0, which, ctrans ? 1 : 0, ysplit);
+                                                            g0, g1, fwd ? 1 : 0, which, ct, pairs, ysplit);
         else
             k_solve_group<<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, dinv, g0, g1,
-                                                      fwd ? 1 : 0, which, ctrans ? 1 : 0);
+                                                      fwd ? 1 : 0, which, ct, pairs);
         AOSX_LAUNCHED();
     };
     auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
         if (nrows == 0) return;
         const int64_t items = nrows * nmat * ngrp;
         trail<<<(unsigned)(8 * ((items + 7) / 8)), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
-                                                                    nt, g0, g1, rows, (int)nrows, nmat,
-                                                                    ctrans ? 1 : 0, ysplit);
+                                                                    nt, g0, g1, rows, (int)nrows, nmat, ct, pairs,
+                                                                    ysplit);
         AOSX_LAUNCHED();
     };
     // Step groups pay off when a launch covers many matrices; with a few, the
@@ -839,14 +840,14 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         const int r = range_begin(RANGE_SOLVE);
         if (!grouped) {
             for (int k = 0; k < nt; ++k) {
-                diag_launch(k, ctrans ? 1 : 0);
+                diag_launch(k, 0);
                 const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
                 if (s) {
                     update_launch(k, d_next + ts.next_ptr[k], s);
                 }
             }
             for (int k = nt - 1; k >= 0; --k) {
-                diag_launch(k, ctrans ? 0 : 1);
+                diag_launch(k, 1);
                 const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
                 if (s) {
                     update_launch(k, d_prev + ts.prev_ptr[k], s);
@@ -857,9 +858,9 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         // group's trailing rows in one launch
         for (int g = 0; grouped && g < ng; ++g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-            if (solve_fold) group_launch(g0, g1, true, ctrans ? 1 : 0);
+            if (solve_fold) group_launch(g0, g1, true, 0);
             for (int k = g0; !solve_fold && k < g1; ++k) {
-                diag_launch(k, ctrans ? 1 : 0);
+                diag_launch(k, 0);
                 const int32_t *nx = ts.next_idx.data() + ts.next_ptr[k];
                 const int s = ts.next_ptr[k + 1] - ts.next_ptr[k];
                 int p = 0;
@@ -873,9 +874,9 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         // backward: the same groups in reverse, steps descending
         for (int g = ng - 1; grouped && g >= 0; --g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
-            if (solve_fold) group_launch(g0, g1, false, ctrans ? 0 : 1);
+            if (solve_fold) group_launch(g0, g1, false, 1);
             for (int k = g1 - 1; !solve_fold && k >= g0; --k) {
-                diag_launch(k, ctrans ? 0 : 1);
+                diag_launch(k, 1);
                 const int32_t *pv = ts.prev_idx.data() + ts.prev_ptr[k];
                 const int s = ts.prev_ptr[k + 1] - ts.prev_ptr[k];
                 int a = s;
@@ -888,14 +889,15 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
         }
         range_end(r, bytes);
     };
-    run_sequence({1, nmat, W, (int64_t)y, (int64_t)tiles, (int64_t)dinv, ctrans ? 1 : 0, precision, (int64_t)ysplit},
-                 launches);
+    run_sequence(
+        {1, nmat, W, (int64_t)y, (int64_t)tiles, (int64_t)dinv, ctrans ? 1 : 0, precision, (int64_t)ysplit, pairs},
+        launches);
 }
 
 void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<int32_t> &rhs_index,
                         const float2 *d_h, const float2 *d_s, const std::vector<int32_t> &kidx,
                         const std::vector<float2> &zeff, float *tiles, float *dinv, bool ctrans, int ir_steps,
-                        float2 *x_out, double *rel_residual, int precision) {
+                        float2 *x_out, double *rel_residual, int precision, int pairs) {
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t padded = (int64_t)nt * 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
@@ -915,7 +917,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
     float2 *y2 = reinterpret_cast<float2 *>(y);
     k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, d_rhs, W, d_ri, d_perm, n, padded);
     AOSX_LAUNCHED();
-    solve(nmat, W, y2, tiles, dinv, ctrans, precision);
+    solve(nmat, W, y2, tiles, dinv, ctrans, precision, pairs);
     k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(x_out, W, y, ystride, nchunk, d_iperm, n);
     AOSX_LAUNCHED();
     if (ir_steps <= 0 && !rel_residual) return;
@@ -939,7 +941,7 @@ void Bundle::solve_full(int nmat, int W, const float2 *d_rhs, const std::vector<
         }
         k_pack_rhs<<<dim3(pb, nmat), 256, 0, stream>>>(y, ystride, nchunk, r, W, d_id, d_perm, n, padded);
         AOSX_LAUNCHED();
-        solve(nmat, W, y2, tiles, dinv, ctrans, precision);
+        solve(nmat, W, y2, tiles, dinv, ctrans, precision, pairs);
         k_unpack<<<dim3(ub, nmat), 256, 0, stream>>>(r, W, y, ystride, nchunk, d_iperm, n);
         AOSX_LAUNCHED();
         k_axpy<<<blocks_for((int64_t)nmat * n * W, 256, 8192), 256, 0, stream>>>(x_out, r, (int64_t)nmat * n * W);
@@ -1419,6 +1421,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             std::vector<int32_t> kidx;
             std::vector<int> shift;
             std::vector<float2> z, zc;
+            // with conjugate pairs, both shifts of every pair in one solve:
+            // the group's k indices twice, its shifts then their conjugates
+            std::vector<int32_t> kidx2;
+            std::vector<float2> z2;
             std::vector<uint8_t> one_product;  // per-shift precision: one BF16 product
             int slot0;
         };
@@ -1438,10 +1444,15 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                     gr.one_product.push_back(std::fabs(std::sin(th)) >= p.far_threshold ? 1 : 0);
                 }
             gr.slot0 = resident ? g * G : 0;
+            gr.kidx2 = gr.kidx;
+            gr.kidx2.insert(gr.kidx2.end(), gr.kidx.begin(), gr.kidx.end());
+            gr.z2 = gr.z;
+            gr.z2.insert(gr.z2.end(), gr.zc.begin(), gr.zc.end());
         }
-        float2 *d_X = buf<float2>("b_X", (size_t)G * n * std::max(L, 32));
+        const int xsets = p.conjugate_pairs ? 2 : 1;  // solution blocks per factorisation
+        float2 *d_X = buf<float2>("b_X", (size_t)xsets * G * n * std::max(L, 32));
         float2 *d_coef = buf<float2>("b_coef", (size_t)G * Ms);
-        auto accumulate = [&](const Group &gr, bool conj_side, float2 *dst, int width, int msize) {
+        auto accumulate = [&](const Group &gr, bool conj_side, float2 *dst, int width, int msize, const float2 *X) {
             // dst[k] += sum over the group's shifts of k of coef * X
             const int per_k = (int)gr.kidx.size() / Kc;
             std::vector<float2> cf(gr.kidx.size() * msize);
@@ -1453,7 +1464,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             AOSX_HIP(hipMemcpyAsync(d_coef, cf.data(), cf.size() * sizeof(float2), hipMemcpyHostToDevice, stream));
             for (int k = 0; k < Kc; ++k) {
                 k_moments<<<blocks_for(n * width, 256, 4096), 256, 0, stream>>>(
-                    dst + (size_t)k * n * width * msize, d_X + (size_t)k * per_k * n * width, per_k, width, msize,
+                    dst + (size_t)k * n * width * msize, X + (size_t)k * per_k * n * width, per_k, width, msize,
                     d_coef + (size_t)k * per_k * msize, n);
                 AOSX_LAUNCHED();
             }
@@ -1472,13 +1483,15 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             double tb = now();
             out.stages[2] += tb - ta;
             out.factorizations += gsz;
-            solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_first, d_X, nullptr,
-                       p.precision);
-            accumulate(gr, false, d_M, L, Ms);
             if (p.conjugate_pairs) {
-                solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_first, d_X, nullptr,
+                solve_full(2 * gsz, L, d_rhs, gr.kidx2, d_H, d_S, gr.kidx2, gr.z2, tl, di, false, ir_first, d_X,
+                           nullptr, p.precision, gsz);
+                accumulate(gr, false, d_M, L, Ms, d_X);
+                accumulate(gr, true, d_M, L, Ms, d_X + (size_t)gsz * n * L);
+            } else {
+                solve_full(gsz, L, d_rhs, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_first, d_X, nullptr,
                            p.precision);
-                accumulate(gr, true, d_M, L, Ms);
+                accumulate(gr, false, d_M, L, Ms, d_X);
             }
             AOSX_HIP(hipStreamSynchronize(stream));
             out.stages[3] += now() - tb;
@@ -1549,7 +1562,7 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             const int64_t y4 = (int64_t)Kc * n * W2 * 8 / 16;
             k_fill_zero<<<blocks_for(y4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(d_Y), y4);
             AOSX_LAUNCHED();
-            d_X = buf<float2>("b_X", (size_t)G * n * W2);
+            d_X = buf<float2>("b_X", (size_t)xsets * G * n * W2);
             for (int g = 0; g < ngroups; ++g) {
                 Group &gr = groups[g];
                 const int gsz = (int)gr.kidx.size();
@@ -1562,13 +1575,15 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
                     out.stages[2] += now() - ta;
                     out.factorizations += gsz;
                 }
-                solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_pass, d_X, nullptr,
-                           p.precision);
-                accumulate(gr, false, d_Y, W2, 1);
                 if (p.conjugate_pairs) {
-                    solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.zc, tl, di, true, ir_pass, d_X,
+                    solve_full(2 * gsz, W2, d_rhs2, gr.kidx2, d_H, d_S, gr.kidx2, gr.z2, tl, di, false, ir_pass, d_X,
+                               nullptr, p.precision, gsz);
+                    accumulate(gr, false, d_Y, W2, 1, d_X);
+                    accumulate(gr, true, d_Y, W2, 1, d_X + (size_t)gsz * n * W2);
+                } else {
+                    solve_full(gsz, W2, d_rhs2, gr.kidx, d_H, d_S, gr.kidx, gr.z, tl, di, false, ir_pass, d_X,
                                nullptr, p.precision);
-                    accumulate(gr, true, d_Y, W2, 1);
+                    accumulate(gr, false, d_Y, W2, 1, d_X);
                 }
                 if (progress) {
                     AOSX_HIP(hipStreamSynchronize(stream));
