@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
     __shared__ int16_t rp[EIGH_MAX / 2], rq[EIGH_MAX / 2];
     __shared__ double part[2][16];
     __shared__ int done;
+    __shared__ double skip2;  // |A(p,q)|^2 below which a pair takes no rotation
     const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
     const int r = rk[k];
     const int rr = r + (r & 1), npairs = rr / 2, nrounds = rr - 1;
@@ -112,6 +113,12 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
                 s += part[1][q];
             }
             done = !(o > tol * tol * s);
+            // threshold Jacobi: an element with |A(p,q)|^2 < tol^2 ||A||_F^2 / r^2
+            // is left alone; once every element is below that, the
+            // off-diagonal norm is below the stopping bound, so the skipped
+            // rotations never hold the loop open, and the late sweeps update
+            // few pair blocks
+            skip2 = tol * tol * s / ((double)r * (double)r);
         }
         __syncthreads();
         if (done) break;
@@ -124,7 +131,7 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
                     const c64 b = S[packed(rr, p, q)];
                     const double mb = sqrt(cabs2(b));
                     const double app = S[packed(rr, p, p)].re, aqq = S[packed(rr, q, q)].re;
-                    if (mb > 1e-300 && mb > 1e-18 * sqrt(fabs(app * aqq))) {
+                    if (mb > 1e-300 && mb * mb >= skip2 && mb > 1e-18 * sqrt(fabs(app * aqq))) {
                         const double tau = (aqq - app) / (2.0 * mb);
                         const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         g.c = 1.0 / sqrt(1.0 + tt * tt);
