@@ -301,21 +301,7 @@ __device__ __forceinline__ void store_acc(float *__restrict__ t, const floatx16 
     }
 }
 
-// Split RHS blocks of the solves (BF16 precisions): four planes of 64 x 64
-// BF16 -- re_hi, re_lo, im_hi, im_lo, with x ~ hi + lo, hi = bf16(x),
-// lo = bf16(x - hi) (round to nearest even) -- stored in MFMA operand order:
-// 1 KiB blocks, one per (K half, plane, 32-row half a, 16-k step within the
-// half), each holding the 32 x 16 operand of one v_mfma_f32_32x32x16_bf16 as
-// lane l of the wave reads it -- row 32a + l % 32, k values 8 (l / 32) .. + 7
-// -- in lane order.
-constexpr int SPLIT_TILE = 4 * TILE2;  // BF16 elements per split RHS block (32 KiB)
-constexpr int SPLIT_HPLANE = TILE * 32;      // one plane of one K half (4 blocks)
-constexpr int SPLIT_HALF = 4 * SPLIT_HPLANE; // all planes of one K half (16 blocks)
-constexpr int SPLIT_BLOCK = 512;             // BF16 per operand block
-__device__ __forceinline__ int split_index(int plane, int row, int k) {
-    return (k >> 5) * SPLIT_HALF + plane * SPLIT_HPLANE + (row >> 5) * 1024 + ((k >> 4) & 1) * SPLIT_BLOCK +
-           ((k >> 3) & 1) * 256 + (row & 31) * 8 + (k & 7);
-}
+constexpr int SPLIT_BLOCK = 512;  // BF16 per MFMA operand block (32 x 16)
 
 __device__ __forceinline__ void split_pair(float x, uint16_t &hi, uint16_t &lo) {
     const __bf16 h = (__bf16)x;
@@ -600,35 +586,10 @@ __global__ void __launch_bounds__(256, 2) k_panel_split(float *__restrict__ tile
     write_panel(Tre, Tim, dst, lower, tid);
 }
 
-// One split RHS block = 2048 chunks of 16 B; thread tid moves chunks
-// q * 256 + tid.  The chunks travel in a register vector (not an array, which
-// the compiler may place in scratch memory when it is carried across loop
-// iterations).
+// Register vector of a thread's share of 32 KiB (8 chunks of 16 B), not an
+// array, which the compiler may place in scratch memory when it is carried
+// across loop iterations.
 typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
-
-__device__ __forceinline__ u32x32 fetch_split(const uint16_t *__restrict__ src, int tid) {
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-    u32x32 r;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const uint4 v = s4[q * 256 + tid];
-        r[4 * q] = v.x;
-        r[4 * q + 1] = v.y;
-        r[4 * q + 2] = v.z;
-        r[4 * q + 3] = v.w;
-    }
-    return r;
-}
-__device__ __forceinline__ void put_split(uint16_t *dst, const u32x32 r, int tid) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = q * 256 + tid;  // 8-element chunk of the split tile
-        const int plane = (e >> 8) & 3, row = ((e >> 7) & 1) * 32 + (e & 31);
-        const int k = (e >> 10) * 32 + ((e >> 6) & 1) * 16 + ((e >> 5) & 1) * 8;
-        *reinterpret_cast<uint4 *>(dst + plane * SPL + row * SLD + k) =
-            make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
-    }
-}
 
 // The re and im planes of a split panel tile (32 KiB of its 48): 2048 chunks
 // of 16 B, chunk e being lane e % 64 of the re/im block e / 64, whose bits

@@ -70,40 +70,67 @@ __global__ void __launch_bounds__(256) k_solve_diag(float *__restrict__ y, int64
 // products, FP32 accumulation; refinement on FP64 residuals removes the
 // difference): the BF16 precisions.
 // Split RHS slots: the finished Y_k of step k is also kept, split and
-// transposed (planes [re_hi, re_lo, im_hi, im_lo] x [column][row]), in slot
-// k % YSLOTS of the matrix's slot area, the B operand layout of the update
-// products; a step group spans at most 32 steps, so its slots are distinct.
+// transposed (rows = RHS columns, k = rows of Y_k) in the six-plane format of
+// the split panel tiles (re, im, re + im, each hi and lo, in MFMA operand
+// blocks), in slot k % YSLOTS of the matrix's slot area: the B operand of the
+// trailing update products, read straight from memory; a step group spans at
+// most 32 steps, so its slots are distinct.
 constexpr int YSLOTS = 64;
 
 __device__ __forceinline__ uint16_t *yslot(uint16_t *ys, int m, int k, int nchunk, int c) {
-    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * SPLIT_TILE;
+    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * PANEL_TILE;
 }
 __device__ __forceinline__ const uint16_t *yslot(const uint16_t *ys, int m, int k, int nchunk, int c) {
-    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * SPLIT_TILE;
+    return ys + (((int64_t)m * YSLOTS + k % YSLOTS) * nchunk + c) * PANEL_TILE;
 }
 
-// Writes the wave's 32 x 32 accumulator block as split, transposed planes:
-// lane l holds column c = l % 32 (of the block), rows 8q + 4(l/32) + 0..3.
-__device__ __forceinline__ void store_acc_split_t(uint16_t *__restrict__ dst, const floatx16 &cre,
-                                                  const floatx16 &cim, int wave, int lane) {
-    const int c = (wave & 1) * 32 + (lane & 31);
+// Accumulator block (a, b) of a 64 x 64 tile (rows 32a.., columns 32b..) in
+// the 32 x 32 MFMA layout: register r of lane l holds row 32a + 8(r/4) +
+// 4(l/32) + r%4, column 32b + l%32.
+__device__ __forceinline__ void load_blk(const float *__restrict__ t, floatx16 &cre, floatx16 &cim, int a, int b,
+                                         int lane) {
+    const int r0 = a * 32 + 4 * (lane >> 5), c = b * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 8 * (r >> 2) + (r & 3);
+        cre[r] = t[row * TILE + c];
+        cim[r] = t[PLANE + row * TILE + c];
+    }
+}
+__device__ __forceinline__ void store_blk(float *__restrict__ t, const floatx16 &cre, const floatx16 &cim, int a,
+                                          int b, int lane) {
+    const int r0 = a * 32 + 4 * (lane >> 5), c = b * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 8 * (r >> 2) + (r & 3);
+        t[row * TILE + c] = cre[r];
+        t[PLANE + row * TILE + c] = cim[r];
+    }
+}
+
+// Writes the wave's 32 x 32 accumulator block (wave / 2, wave % 2) of Y_k into
+// its slot: transposed (row = column of Y_k), six planes, panel block order.
+// Lane l holds column c = l % 32 of the block and 4 consecutive rows (k
+// values) 8q + 4(l/32) .. + 3: 8 bytes per plane.
+__device__ __forceinline__ void store_acc_slot(uint16_t *__restrict__ dst, const floatx16 &cre, const floatx16 &cim,
+                                               int wave, int lane) {
+    const int a = wave & 1, col = lane & 31;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int row = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;
-        uint16_t rh[4], rl[4], ih[4], il[4];
+        const int k0 = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;  // first of the 4 rows of Y_k
+        uint16_t v[6][4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            split_pair(cre[4 * q + e], rh[e], rl[e]);
-            split_pair(cim[4 * q + e], ih[e], il[e]);
+            const float xr = cre[4 * q + e], xi = cim[4 * q + e];
+            split_pair(xr, v[0][e], v[1][e]);
+            split_pair(xi, v[2][e], v[3][e]);
+            split_pair(xr + xi, v[4][e], v[5][e]);
         }
-        const int off = split_index(0, c, row);  // 4 consecutive k within one half
-        auto pack = [](const uint16_t *v) {
-            return make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
-        };
-        *reinterpret_cast<uint2 *>(dst + off) = pack(rh);
-        *reinterpret_cast<uint2 *>(dst + SPLIT_HPLANE + off) = pack(rl);
-        *reinterpret_cast<uint2 *>(dst + 2 * SPLIT_HPLANE + off) = pack(ih);
-        *reinterpret_cast<uint2 *>(dst + 3 * SPLIT_HPLANE + off) = pack(il);
+        const int lane_t = ((k0 >> 3) & 1) * 32 + col, off = lane_t * 8 + (k0 & 7);
+#pragma unroll
+        for (int pl = 0; pl < 6; ++pl)
+            *reinterpret_cast<uint2 *>(dst + panel_block(k0 >> 4, pl >> 1, pl & 1, a) * SPLIT_BLOCK + off) =
+                make_uint2(v[pl][0] | ((uint32_t)v[pl][1] << 16), v[pl][2] | ((uint32_t)v[pl][3] << 16));
     }
 }
 
@@ -124,7 +151,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_diag_split(float *__restrict__
         floatx16 cre = {}, cim = {};
         mma_split<true>(As, Bs, cre, cim, wave, lane);
         store_acc(yk, cre, cim, wave, lane);
-        store_acc_split_t(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
+        store_acc_slot(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
     }
 }
 
@@ -175,9 +202,9 @@ __global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict
     stage_split(t, As, ctrans != 0, true, ctrans != 0, tid);
     for (int c = 0; c < nchunk; ++c) {
         float *yi = y + rhs_block(y_mat_stride, m, i, nchunk, c);
-        const u32x32 yv = fetch_split(yslot(ysplit, m, k, nchunk, c), tid);
+        const u32x32 yv = fetch_panel4(yslot(ysplit, m, k, nchunk, c), tid);
         __syncthreads();
-        put_split(Bs, yv, tid);
+        put_panel4(Bs, yv, tid);
         floatx16 cre, cim;
         load_acc(yi, cre, cim, wave, lane);
         __syncthreads();
@@ -194,18 +221,15 @@ __global__ void __launch_bounds__(256, 2) k_solve_update_split(float *__restrict
 // so Y_i is read and written once per group instead of once per step.  Grid:
 // 8 * ceil(nrows * nmat * ngrp / 8) blocks in one dimension, mapped to items
 // (matrix, chunk group, row) so that each XCD takes a contiguous run (rows of
-// one matrix share the group's Y_k blocks in its L2).
-template <bool SPLIT, int NC>
+// one matrix share the group's Y_k blocks in its L2).  FP32 MFMA form.
+template <int NC>
 __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
                                                        const float *__restrict__ tiles, int64_t mat_stride,
                                                        const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
                                                        const int32_t *__restrict__ rows, int nrows, int nmat,
-                                                       int ctrans, int pairs, const uint16_t *__restrict__ ysplit) {
-    // FP32 operands (Are/Aim/Bre/Bim) or split operands (As/Bs) share one
-    // LDS area.
-    __shared__ __attribute__((aligned(16))) float lds[SPLIT ? (8 * SPL) / 2 : 2 * TILE * LDA + 2 * TILE2];
-    float *Are = lds, *Aim = lds + TILE * LDA, *Bre = lds + 2 * TILE * LDA, *Bim = Bre + TILE2;
-    uint16_t *As = reinterpret_cast<uint16_t *>(lds), *Bs = As + 4 * SPL;
+                                                       int ctrans, int pairs) {
+    __shared__ float Are[TILE * LDA], Aim[TILE * LDA];
+    __shared__ __attribute__((aligned(16))) float Bre[TILE2], Bim[TILE2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ngrp = (nchunk + NC - 1) / NC;
     const int64_t total = (int64_t)nrows * nmat * ngrp, per = (total + 7) / 8;
@@ -235,47 +259,7 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
 #pragma unroll
     for (int c = 0; c < NC; ++c)
         if (c < nc) load_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
-    if (SPLIT) {
-        // the next present tile and its first RHS block are loaded into
-        // registers while the current products run
-        const float *tm = tiles + (int64_t)st.fac * mat_stride;
-        int k = next_k(g0);
-        TileBlock tn;
-        u32x32 yn;
-        if (k < g1) {
-            tn = tile_block_load(tm + (int64_t)tile_of(k) * TILE_FLOATS, tid);
-            yn = fetch_split(yslot(ysplit, m, k, nchunk, c0), tid);
-        }
-        // The barriers order LDS only, so the loads issued for the next tile
-        // (and this step's other RHS chunks) stay in flight across them and
-        // land while the products run.
-        while (k < g1) {
-            const int kn = next_k(k + 1);
-            lds_barrier();  // the previous products are done with LDS
-            stage_split_block(tn, As, ctrans != 0, true, ctrans != 0, tid);
-            put_split(Bs, yn, tid);
-            u32x32 ym[NC > 1 ? NC - 1 : 1];
-#pragma unroll
-            for (int c = 1; c < NC; ++c)
-                if (c < nc) ym[c - 1] = fetch_split(yslot(ysplit, m, k, nchunk, c0 + c), tid);
-            if (kn < g1) {
-                tn = tile_block_load(tm + (int64_t)tile_of(kn) * TILE_FLOATS, tid);
-                yn = fetch_split(yslot(ysplit, m, kn, nchunk, c0), tid);
-            }
-            lds_barrier();
-            mma_split<true>(As, Bs, cre[0], cim[0], wave, lane);
-#pragma unroll
-            for (int c = 1; c < NC; ++c) {
-                if (c >= nc) break;
-                lds_barrier();
-                put_split(Bs, ym[c - 1], tid);
-                lds_barrier();
-                mma_split<true>(As, Bs, cre[c], cim[c], wave, lane);
-            }
-            k = kn;
-        }
-    }
-    for (int k = next_k(g0); !SPLIT && k < g1; k = next_k(k + 1)) {
+    for (int k = next_k(g0); k < g1; k = next_k(k + 1)) {
         const int32_t id = tile_of(k);
         const float *t = tiles + (int64_t)st.fac * mat_stride + (int64_t)id * TILE_FLOATS;
         __syncthreads();  // the previous products are done with LDS
@@ -294,6 +278,166 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail(float *__restrict__ y, i
     for (int c = 0; c < NC; ++c)
         if (c < nc) store_acc(y + rhs_block(y_mat_stride, m, i, nchunk, c0 + c), cre[c], cim[c], wave, lane);
 }
+
+template __global__ void k_solve_trail<2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
+                                          int, const int32_t *, int, int, int, int);
+
+// Stages thread tid's 4 x 4 block of an FP32 tile as the A operand P of a
+// three-multiplication product in six planes, panel block order (row = M
+// index, k = K index): P = -T (transpose = false) or -T^H (transpose and
+// conj): per P row 4 consecutive k values, 8 bytes per plane.
+__device__ __forceinline__ void stage_panel_block(const TileBlock &blk, uint16_t *S, bool transpose, bool conj,
+                                                  int tid) {
+    const int br = tid >> 4, bc = tid & 15;
+    float re[4][4], im[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 vr = blk.re[q], vi = blk.im[q];
+        re[q][0] = vr.x; re[q][1] = vr.y; re[q][2] = vr.z; re[q][3] = vr.w;
+        im[q][0] = vi.x; im[q][1] = vi.y; im[q][2] = vi.z; im[q][3] = vi.w;
+    }
+    const float si = conj ? 1.f : -1.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int row = transpose ? 4 * bc + u : 4 * br + u;
+        const int k0 = transpose ? 4 * br : 4 * bc;
+        uint16_t v[6][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float xr = -(transpose ? re[e][u] : re[u][e]);
+            const float xi = si * (transpose ? im[e][u] : im[u][e]);
+            split_pair(xr, v[0][e], v[1][e]);
+            split_pair(xi, v[2][e], v[3][e]);
+            split_pair(xr + xi, v[4][e], v[5][e]);
+        }
+        const int off = (((k0 >> 3) & 1) * 32 + (row & 31)) * 8 + (k0 & 7);
+#pragma unroll
+        for (int pl = 0; pl < 6; ++pl)
+            *reinterpret_cast<uint2 *>(S + panel_block(k0 >> 4, pl >> 1, pl & 1, row >> 5) * SPLIT_BLOCK + off) =
+                make_uint2(v[pl][0] | ((uint32_t)v[pl][1] << 16), v[pl][2] | ((uint32_t)v[pl][3] << 16));
+    }
+}
+
+// The same trailing update in the BF16 precisions, by the three-
+// multiplication complex product: Y_i += P Y_k with P = -T_ik (or -T_ki^H),
+//   T1 = Pr Yr, T2 = Pi Yi, T3 = (Pr + Pi)(Yr + Yi),
+//   Re Y_i += T1 - T2,  Im Y_i += T3 - T1 - T2,
+// each real product as hi*hi + hi*lo + lo*hi (three accumulators per 32 x 32
+// block, started at x1 = Re, x2 = 0, x3 = Re + Im).  P is split once per step
+// into LDS (six planes, 48 KiB) from the FP32 tile, which is loaded into
+// registers a step ahead; Y_k comes split, transposed and in MFMA block order
+// from its slot, straight from memory into registers, one K quarter ahead, so
+// a step takes two barriers.  NC = 2: wave w computes the 64 rows x 32 columns
+// of column half w % 2 of chunk c0 + w / 2 (two 32 x 32 blocks); NC = 1: the
+// 32 x 32 block (w / 2, w % 2) of chunk c0.
+template <int NC>
+__global__ void __launch_bounds__(256, 2) k_solve_trail_3m(float *__restrict__ y, int64_t y_mat_stride, int nchunk,
+                                                          const float *__restrict__ tiles, int64_t mat_stride,
+                                                          const int32_t *__restrict__ tile_id, int nt, int g0, int g1,
+                                                          const int32_t *__restrict__ rows, int nrows, int nmat,
+                                                          int ctrans, int pairs, const uint16_t *__restrict__ ysplit) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[PANEL_TILE];
+    constexpr int NB = NC == 2 ? 2 : 1;  // 32 x 32 output blocks per wave
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int ngrp = (nchunk + NC - 1) / NC;
+    const int64_t total = (int64_t)nrows * nmat * ngrp, per = (total + 7) / 8;
+    const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= total) return;
+    const int r = (int)(item % nrows);
+    const int64_t mc = item / nrows;
+    const int m = (int)(mc / ngrp), c0 = (int)(mc % ngrp) * NC;
+    const SolveSet st = solve_set(m, pairs, ctrans);
+    const int ct = st.ct;
+    const int i = rows[r];
+    const int wc = NC == 2 ? wave >> 1 : 0, wa = NC == 2 ? 0 : wave >> 1, wb = wave & 1;
+    const int c = c0 + wc;
+    const bool active = c < nchunk;  // NC = 2 and an odd chunk count: the last group's second chunk is absent
+    const int G = g1 - g0;
+    const int32_t lane_id = lane >= G ? -1
+                            : ct ? tile_id[(int64_t)(g0 + lane) * nt + i] : tile_id[(int64_t)i * nt + g0 + lane];
+    const uint32_t present = (uint32_t)__ballot(lane_id >= 0);
+    auto tile_of = [&](int k) { return __builtin_amdgcn_readlane(lane_id, k - g0); };
+    auto next_k = [&](int k) {
+        const int j = k - g0;
+        const uint32_t rest = j >= 32 ? 0u : present & (~0u << j);
+        return rest ? g0 + __builtin_ctz(rest) : g1;
+    };
+    float *Yi = y + rhs_block(y_mat_stride, m, i, nchunk, active ? c : c0);
+    floatx16 x[3][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (active) {
+            load_blk(Yi, x[0][b], x[2][b], wa + b, wb, lane);
+            x[2][b] += x[0][b];
+        } else {
+            x[0][b] = x[2][b] = floatx16{};
+        }
+        x[1][b] = floatx16{};
+    }
+    const float *tm = tiles + (int64_t)st.fac * mat_stride;
+    int k = next_k(g0);
+    TileBlock tn;
+    if (k < g1) tn = tile_block_load(tm + (int64_t)tile_of(k) * TILE_FLOATS, tid);
+    // B operands of K quarter q of step kk: blocks (q, component, part, wb)
+    // of the slot of Y_kk, chunk c
+    u32x4 bq[2][3][2];
+    auto fetch_b = [&](u32x4 (&dst)[3][2], int kk, int q) {
+        const uint16_t *sl = yslot(ysplit, m, kk, nchunk, c) + lane * 8;
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                dst[comp][h] = *reinterpret_cast<const u32x4 *>(sl + panel_block(q, comp, h, wb) * SPLIT_BLOCK);
+    };
+    if (k < g1 && active) fetch_b(bq[0], k, 0);
+    while (k < g1) {
+        const int kn = next_k(k + 1);
+        lds_barrier();  // the previous step's reads of As are done
+        stage_panel_block(tn, As, ct != 0, ct != 0, tid);
+        if (kn < g1) tn = tile_block_load(tm + (int64_t)tile_of(kn) * TILE_FLOATS, tid);
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                // the next quarter's B operands (the next step's first quarter
+                // after the last), loading while this quarter's products run
+                if (q < 3) fetch_b(bq[(q + 1) & 1], k, q + 1);
+                else if (kn < g1) fetch_b(bq[0], kn, 0);
+#pragma unroll
+                for (int comp = 0; comp < 3; ++comp) {
+                    bf16x8 ah[NB], al[NB];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        ah[b] = lds8(As + panel_block(q, comp, 0, wa + b) * SPLIT_BLOCK + lane * 8);
+                        al[b] = lds8(As + panel_block(q, comp, 1, wa + b) * SPLIT_BLOCK + lane * 8);
+                    }
+                    const bf16x8 bh = __builtin_bit_cast(bf16x8, bq[q & 1][comp][0]);
+                    const bf16x8 bl = __builtin_bit_cast(bf16x8, bq[q & 1][comp][1]);
+#pragma unroll
+                    for (int pr = 0; pr < 3; ++pr)
+#pragma unroll
+                        for (int b = 0; b < NB; ++b)
+                            x[comp][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr == 2 ? al[b] : ah[b],
+                                                                                 pr == 1 ? bl : bh, x[comp][b], 0, 0,
+                                                                                 0);
+                }
+            }
+        }
+        k = kn;
+    }
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const floatx16 re = x[0][b] - x[1][b], im = x[2][b] - x[0][b] - x[1][b];
+            store_blk(Yi, re, im, wa + b, wb, lane);
+        }
+    }
+}
+
+template __global__ void k_solve_trail_3m<2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
+                                             int, const int32_t *, int, int, int, int, const uint16_t *);
+template __global__ void k_solve_trail_3m<1>(float *, int64_t, int, const float *, int64_t, const int32_t *, int, int,
+                                             int, const int32_t *, int, int, int, int, const uint16_t *);
 
 // The steps of one step group [g0, g1) of a sweep in one launch: block
 // (matrix m, chunk c) takes the group's steps in sweep order (ascending when
@@ -445,12 +589,12 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
         const uint16_t *B = Bs;
         if (diag) acc_to_split(Bs, cre, cim, wave, lane);
         else if (u == s - 1) B = Ys;  // the step finished last
-        else put_split(Bs, bn, tid);
+        else put_panel4(Bs, bn, tid);
         // the next op's operands: finished steps other than the last are in
         // their slots (stored before an earlier full barrier)
         if (s2 < G) {
             an = tile_block_load(a_tile(s2, u2), tid);
-            if (u2 < s2 - 1) bn = fetch_split(yslot(ysplit, m, step_of(u2), nchunk, c), tid);
+            if (u2 < s2 - 1) bn = fetch_panel4(yslot(ysplit, m, step_of(u2), nchunk, c), tid);
         }
         floatx16 nre = {}, nim = {};
         if (diag && s + 1 < G) load_acc(y + rhs_block(y_mat_stride, m, step_of(s + 1), nchunk, c), nre, nim, wave, lane);
@@ -463,7 +607,7 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
         if (diag) {
             const int k = step_of(s);
             store_acc(y + rhs_block(y_mat_stride, m, k, nchunk, c), cre, cim, wave, lane);
-            store_acc_split_t(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
+            store_acc_slot(yslot(ysplit, m, k, nchunk, c), cre, cim, wave, lane);
             // Ys: its last readers (the updates of step s by step s - 1) ran
             // before this op's first barrier
             acc_to_split(Ys, cre, cim, wave, lane);
@@ -476,10 +620,6 @@ __global__ void __launch_bounds__(256, 1) k_solve_group_split(float *__restrict_
     }
 }
 
-template __global__ void k_solve_trail<false, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                 int, int, const int32_t *, int, int, int, int, const uint16_t *);
-template __global__ void k_solve_trail<true, 2>(float *, int64_t, int, const float *, int64_t, const int32_t *, int,
-                                                int, int, const int32_t *, int, int, int, int, const uint16_t *);
 
 // Y[m] (tiled planar, W = 64 * nchunk columns) <- src[src_index[m]]
 // (row-major n x w_src complex64); columns >= w_src and padded rows are zero.

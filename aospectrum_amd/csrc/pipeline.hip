@@ -785,9 +785,8 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
     const int nt = ts.n_diag, nchunk = (W + 63) / 64;
     const int64_t ystride = (int64_t)nt * nchunk * TILE_FLOATS;
     const int64_t mstride = (int64_t)factor_floats();
-    auto trail = split ? k_solve_trail<true, 2> : k_solve_trail<false, 2>;
     // split RHS slots (BF16 precisions): finished Y_k blocks, split once
-    uint16_t *ysplit = split ? buf<uint16_t>("s_ysplit", (size_t)nmat * YSLOTS * nchunk * SPLIT_TILE) : nullptr;
+    uint16_t *ysplit = split ? buf<uint16_t>("s_ysplit", (size_t)nmat * YSLOTS * nchunk * PANEL_TILE) : nullptr;
     // Y_k <- D_k Y_k (which: 0 inv(L_kk), 1 inv(U_kk) for a plain set; a
     // conjugate set takes the other), and the split copy
     const int ct = ctrans ? 1 : 0;
@@ -808,7 +807,10 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
                                                                    k, rows_dev, ct, pairs);
         AOSX_LAUNCHED();
     };
-    const int ngrp = (nchunk + 1) / 2;
+    // chunks per trailing block: two, or one for a single-chunk RHS in the
+    // BF16 precisions (its four waves then take one 32 x 32 block each)
+    const int tnc = split && nchunk == 1 ? 1 : 2;
+    const int ngrp = (nchunk + tnc - 1) / tnc;
     const int ng = (int)grp_begin.size() - 1;
     // the steps of group [g0, g1) in one launch (see k_solve_group)
     auto group_launch = [&](int g0, int g1, bool fwd, int which) {
@@ -824,9 +826,16 @@ void Bundle::solve(int nmat, int W, float2 *y2, float *tiles, float *dinv, bool 
     auto trailing = [&](int g0, int g1, const int32_t *rows, int64_t nrows) {
         if (nrows == 0) return;
         const int64_t items = nrows * nmat * ngrp;
-        trail<<<(unsigned)(8 * ((items + 7) / 8)), 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id,
-                                                                    nt, g0, g1, rows, (int)nrows, nmat, ct, pairs,
-                                                                    ysplit);
+        const unsigned blocks = (unsigned)(8 * ((items + 7) / 8));
+        if (!split)
+            k_solve_trail<2><<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, g0, g1,
+                                                         rows, (int)nrows, nmat, ct, pairs);
+        else if (tnc == 2)
+            k_solve_trail_3m<2><<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, g0,
+                                                            g1, rows, (int)nrows, nmat, ct, pairs, ysplit);
+        else
+            k_solve_trail_3m<1><<<blocks, 256, 0, stream>>>(y, ystride, nchunk, tiles, mstride, d_tile_id, nt, g0,
+                                                            g1, rows, (int)nrows, nmat, ct, pairs, ysplit);
         AOSX_LAUNCHED();
     };
     // Step groups pay off when a launch covers many matrices; with a few, the
