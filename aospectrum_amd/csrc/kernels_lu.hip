@@ -303,11 +303,25 @@ __device__ __forceinline__ void store_acc(float *__restrict__ t, const floatx16 
 
 constexpr int SPLIT_BLOCK = 512;  // BF16 per MFMA operand block (32 x 16)
 
-__device__ __forceinline__ void split_pair(float x, uint16_t &hi, uint16_t &lo) {
-    const __bf16 h = (__bf16)x;
-    const __bf16 l = (__bf16)(x - (float)h);
-    hi = __builtin_bit_cast(uint16_t, h);
-    lo = __builtin_bit_cast(uint16_t, l);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// BF16 hi / lo parts of two FP32 values, packed (x0 in the low half): hi =
+// bf16(x), lo = bf16(x - hi), both rounded to nearest even.  One packed
+// conversion per part covers both values.
+__device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi, uint32_t &lo) {
+    const floatx2 x = {x0, x1};
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+    const floatx2 hf = {__builtin_bit_cast(float, h << 16), __builtin_bit_cast(float, h & 0xffff0000u)};
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - hf, bf16x2));
+    hi = h;
+}
+// The six split parts (re, im, re + im; hi, lo) of two complex values, as
+// split2 packs them.
+__device__ __forceinline__ void split6(float xr0, float xi0, float xr1, float xi1, uint32_t w[6]) {
+    split2(xr0, xr1, w[0], w[1]);
+    split2(xi0, xi1, w[2], w[3]);
+    split2(xr0 + xi0, xr1 + xi1, w[4], w[5]);
 }
 
 // Split panel tiles (BF16 precisions).  The panels of a step group keep each
@@ -336,10 +350,8 @@ __device__ __forceinline__ int panel_block(int q, int c, int h, int a) { return 
 __device__ __forceinline__ void write_panel(const float *Tre, const float *Tim, uint16_t *__restrict__ dst,
                                             bool negate_rows, int tid) {
     const int r = tid >> 2, q = tid & 3, c0 = 16 * q;
-    uint16_t v[6][16];  // [component * 2 + part][k]
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        float xr, xi;
+    uint32_t w[8][6];  // [k pair][component * 2 + part]
+    auto ld = [&](int c, float &xr, float &xi) {
         if (negate_rows) {
             xr = -Tre[r * LDA + c0 + c];
             xi = -Tim[r * LDA + c0 + c];
@@ -347,19 +359,19 @@ __device__ __forceinline__ void write_panel(const float *Tre, const float *Tim, 
             xr = Tre[(c0 + c) * LDA + r];
             xi = Tim[(c0 + c) * LDA + r];
         }
-        split_pair(xr, v[0][c], v[1][c]);
-        split_pair(xi, v[2][c], v[3][c]);
-        split_pair(xr + xi, v[4][c], v[5][c]);
-    }
-    auto pack = [](const uint16_t *w) {
-        return make_uint4(w[0] | ((uint32_t)w[1] << 16), w[2] | ((uint32_t)w[3] << 16),
-                          w[4] | ((uint32_t)w[5] << 16), w[6] | ((uint32_t)w[7] << 16));
     };
+#pragma unroll
+    for (int c = 0; c < 16; c += 2) {
+        float xr0, xi0, xr1, xi1;
+        ld(c, xr0, xi0);
+        ld(c + 1, xr1, xi1);
+        split6(xr0, xi0, xr1, xi1, w[c / 2]);
+    }
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
         uint16_t *b = dst + panel_block(q, p >> 1, p & 1, r >> 5) * SPLIT_BLOCK + (r & 31) * 8;
-        *reinterpret_cast<uint4 *>(b) = pack(v[p]);            // k 16q + 0..7: lanes 0..31
-        *reinterpret_cast<uint4 *>(b + 256) = pack(v[p] + 8);  // k 16q + 8..15: lanes 32..63
+        *reinterpret_cast<uint4 *>(b) = make_uint4(w[0][p], w[1][p], w[2][p], w[3][p]);        // k 16q + 0..7
+        *reinterpret_cast<uint4 *>(b + 256) = make_uint4(w[4][p], w[5][p], w[6][p], w[7][p]);  // k 16q + 8..15
     }
 }
 
@@ -521,22 +533,22 @@ __device__ __forceinline__ void stage_split_block(const TileBlock &blk, uint16_t
         // run u: S row and its 4 consecutive K values
         const int row = transpose ? 4 * bc + u : 4 * br + u;
         const int k0 = transpose ? 4 * br : 4 * bc;
-        uint16_t rh[4], rl[4], ih[4], il[4];
+        float xr[4], xi[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float xr = sr * (transpose ? re[e][u] : re[u][e]);
-            const float xi = si * (transpose ? im[e][u] : im[u][e]);
-            split_pair(xr, rh[e], rl[e]);
-            split_pair(xi, ih[e], il[e]);
+            xr[e] = sr * (transpose ? re[e][u] : re[u][e]);
+            xi[e] = si * (transpose ? im[e][u] : im[u][e]);
         }
+        uint2 rh, rl, ih, il;
+        split2(xr[0], xr[1], rh.x, rl.x);
+        split2(xr[2], xr[3], rh.y, rl.y);
+        split2(xi[0], xi[1], ih.x, il.x);
+        split2(xi[2], xi[3], ih.y, il.y);
         const int off = row * SLD + k0;
-        *reinterpret_cast<uint2 *>(S + off) = make_uint2(rh[0] | ((uint32_t)rh[1] << 16), rh[2] | ((uint32_t)rh[3] << 16));
-        *reinterpret_cast<uint2 *>(S + SPL + off) =
-            make_uint2(rl[0] | ((uint32_t)rl[1] << 16), rl[2] | ((uint32_t)rl[3] << 16));
-        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) =
-            make_uint2(ih[0] | ((uint32_t)ih[1] << 16), ih[2] | ((uint32_t)ih[3] << 16));
-        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) =
-            make_uint2(il[0] | ((uint32_t)il[1] << 16), il[2] | ((uint32_t)il[3] << 16));
+        *reinterpret_cast<uint2 *>(S + off) = rh;
+        *reinterpret_cast<uint2 *>(S + SPL + off) = rl;
+        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) = ih;
+        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) = il;
     }
 }
 __device__ __forceinline__ void stage_split(const float *__restrict__ g, uint16_t *S, bool transpose, bool negate,

@@ -118,19 +118,14 @@ __device__ __forceinline__ void store_acc_slot(uint16_t *__restrict__ dst, const
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k0 = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;  // first of the 4 rows of Y_k
-        uint16_t v[6][4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const float xr = cre[4 * q + e], xi = cim[4 * q + e];
-            split_pair(xr, v[0][e], v[1][e]);
-            split_pair(xi, v[2][e], v[3][e]);
-            split_pair(xr + xi, v[4][e], v[5][e]);
-        }
+        uint32_t w0[6], w1[6];
+        split6(cre[4 * q], cim[4 * q], cre[4 * q + 1], cim[4 * q + 1], w0);
+        split6(cre[4 * q + 2], cim[4 * q + 2], cre[4 * q + 3], cim[4 * q + 3], w1);
         const int lane_t = ((k0 >> 3) & 1) * 32 + col, off = lane_t * 8 + (k0 & 7);
 #pragma unroll
         for (int pl = 0; pl < 6; ++pl)
             *reinterpret_cast<uint2 *>(dst + panel_block(k0 >> 4, pl >> 1, pl & 1, a) * SPLIT_BLOCK + off) =
-                make_uint2(v[pl][0] | ((uint32_t)v[pl][1] << 16), v[pl][2] | ((uint32_t)v[pl][3] << 16));
+                make_uint2(w0[pl], w1[pl]);
     }
 }
 
@@ -301,20 +296,20 @@ __device__ __forceinline__ void stage_panel_block(const TileBlock &blk, uint16_t
     for (int u = 0; u < 4; ++u) {
         const int row = transpose ? 4 * bc + u : 4 * br + u;
         const int k0 = transpose ? 4 * br : 4 * bc;
-        uint16_t v[6][4];
+        float xr[4], xi[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float xr = -(transpose ? re[e][u] : re[u][e]);
-            const float xi = si * (transpose ? im[e][u] : im[u][e]);
-            split_pair(xr, v[0][e], v[1][e]);
-            split_pair(xi, v[2][e], v[3][e]);
-            split_pair(xr + xi, v[4][e], v[5][e]);
+            xr[e] = -(transpose ? re[e][u] : re[u][e]);
+            xi[e] = si * (transpose ? im[e][u] : im[u][e]);
         }
+        uint32_t w0[6], w1[6];
+        split6(xr[0], xi[0], xr[1], xi[1], w0);
+        split6(xr[2], xi[2], xr[3], xi[3], w1);
         const int off = (((k0 >> 3) & 1) * 32 + (row & 31)) * 8 + (k0 & 7);
 #pragma unroll
         for (int pl = 0; pl < 6; ++pl)
             *reinterpret_cast<uint2 *>(S + panel_block(k0 >> 4, pl >> 1, pl & 1, row >> 5) * SPLIT_BLOCK + off) =
-                make_uint2(v[pl][0] | ((uint32_t)v[pl][1] << 16), v[pl][2] | ((uint32_t)v[pl][3] << 16));
+                make_uint2(w0[pl], w1[pl]);
     }
 }
 
@@ -507,20 +502,16 @@ __device__ __forceinline__ void acc_to_split(uint16_t *S, const floatx16 &cre, c
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int row = (wave >> 1) * 32 + 4 * (lane >> 5) + 8 * q;
-        uint16_t rh[4], rl[4], ih[4], il[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            split_pair(cre[4 * q + e], rh[e], rl[e]);
-            split_pair(cim[4 * q + e], ih[e], il[e]);
-        }
-        auto pack = [](const uint16_t *v) {
-            return make_uint2(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16));
-        };
+        uint2 rh, rl, ih, il;
+        split2(cre[4 * q], cre[4 * q + 1], rh.x, rl.x);
+        split2(cre[4 * q + 2], cre[4 * q + 3], rh.y, rl.y);
+        split2(cim[4 * q], cim[4 * q + 1], ih.x, il.x);
+        split2(cim[4 * q + 2], cim[4 * q + 3], ih.y, il.y);
         const int off = col * SLD + row;
-        *reinterpret_cast<uint2 *>(S + off) = pack(rh);
-        *reinterpret_cast<uint2 *>(S + SPL + off) = pack(rl);
-        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) = pack(ih);
-        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) = pack(il);
+        *reinterpret_cast<uint2 *>(S + off) = rh;
+        *reinterpret_cast<uint2 *>(S + SPL + off) = rl;
+        *reinterpret_cast<uint2 *>(S + 2 * SPL + off) = ih;
+        *reinterpret_cast<uint2 *>(S + 3 * SPL + off) = il;
     }
 }
 

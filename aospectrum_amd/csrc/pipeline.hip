@@ -1057,7 +1057,7 @@ static void eigh_batched(Bundle &B, const c64 *A, int ld, int64_t stride, const 
 static void compress_and_project(Bundle &B, int nk, const float2 *d_M, int P, const float2 *d_H,
                                  const float2 *d_S, const CissParams &p, const double *lower, const double *upper,
                                  Subspace &out, double *stage, const char *ritz_name, bool want_vectors = false) {
-    const int64_t n = B.n, nnz = B.nnz;
+    const int64_t n = B.n;
     hipStream_t st = B.stream;
     double t0 = now();
     std::vector<int32_t> iota_k(nk);
