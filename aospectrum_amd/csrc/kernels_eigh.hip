@@ -11,12 +11,15 @@
 //                     triangle -- in LDS when rr <= 128 (132 KiB), else in a
 //                     global scratch -- and each round updates every 2 x 2
 //                     pair block B(a, b) <- M_a^H B M_b once (rows and columns
-//                     together, each element read and written once).  The
-//                     rotations are logged; the eigenvalues come out ascending
-//                     with the rank of every column.
+//                     together, each element read and written once): the
+//                     diagonal blocks by the threads that compute the
+//                     rotations, the off-diagonal ones by the whole
+//                     workgroup.  The rotations are logged; the eigenvalues
+//                     come out ascending with the rank of every column.
 //   k_jacobi_vectors  rows of V are independent under column rotations, so V
-//                     is rebuilt from the log by many workgroups, 8 rows each,
-//                     held in LDS; the log is staged 8 rounds at a time.
+//                     is rebuilt from the log one row per wave, held in LDS;
+//                     the log is staged 8 rounds at a time, the next batch
+//                     loading while one is applied.
 #include "device_common.h"
 
 namespace aosx {
@@ -29,19 +32,17 @@ struct Rot {
     double c, s, er, ei;  // e^{i phi} = er + i ei
 };
 
+// pair a of round t (t < rr - 1, a < rr / 2), p < q:
+//   a = 0: (rr - 1, t);  a > 0: ((t + a) mod (rr - 1), (t - a) mod (rr - 1))
 __device__ __forceinline__ void rr_pair(int rr, int t, int a, int &p, int &q) {
-    if (a == 0) {
-        p = rr - 1;
-        q = t;
-    } else {
-        p = (t + a) % (rr - 1);
-        q = (t - a + (rr - 1)) % (rr - 1);
+    const int m = rr - 1;
+    int x = m, y = t;
+    if (a > 0) {
+        x = t + a >= m ? t + a - m : t + a;
+        y = t - a < 0 ? t - a + m : t - a;
     }
-    if (p > q) {
-        const int x = p;
-        p = q;
-        q = x;
-    }
+    p = min(x, y);
+    q = max(x, y);
 }
 
 __device__ __forceinline__ int packed(int rr, int i, int j) {  // i <= j
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
     const int k = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
     const int r = rk[k];
     const int rr = r + (r & 1), npairs = rr / 2, nrounds = rr - 1;
-    const int nblocks = npairs * (npairs + 1) / 2;
+    const int nblocks = npairs * (npairs - 1) / 2;  // off-diagonal pair blocks
     c64 *S = IN_LDS ? Sl : scratch + (int64_t)k * (EIGH_MAX * (EIGH_MAX + 1) / 2);
     const c64 *A = A_all + k * stride;
     Rot *lg = log + (int64_t)k * log_stride;
@@ -81,14 +82,37 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
         S[packed(rr, i, j)] = (i < r && j < r) ? A[(int64_t)i * ld + j] : c64{0.0, 0.0};
     }
     for (int a = tid; a < npairs; a += nthr) {
-        const int base = a * npairs - ((a * (a - 1)) >> 1);
-        for (int b = a; b < npairs; ++b) {
-            blk_a[base + b - a] = (uint8_t)a;
-            blk_b[base + b - a] = (uint8_t)b;
+        const int base = a * (npairs - 1) - ((a * (a - 1)) >> 1);  // blocks (a', b), a' < a, b > a'
+        for (int b = a + 1; b < npairs; ++b) {
+            blk_a[base + b - a - 1] = (uint8_t)a;
+            blk_b[base + b - a - 1] = (uint8_t)b;
         }
     }
     __syncthreads();
-    auto at = [&](int i, int j) -> c64 { return i <= j ? S[packed(rr, i, j)] : cconj(S[packed(rr, j, i)]); };
+    // element (i, j) of the Hermitian matrix, and its store, through the
+    // packed upper triangle (address and conjugation by selects, no branches)
+    auto at = [&](int i, int j) -> c64 {
+        const c64 v = S[packed(rr, min(i, j), max(i, j))];
+        return {v.re, i <= j ? v.im : -v.im};
+    };
+    auto put = [&](int i, int j, c64 v) { S[packed(rr, min(i, j), max(i, j))] = {v.re, i <= j ? v.im : -v.im}; };
+    // pair block B = [[A(pa,pb), A(pa,qb)], [A(qa,pb), A(qa,qb)]] <- M_a^H B M_b
+    auto rotate = [](const Rot &ga, const Rot &gb, c64 &b00, c64 &b01, c64 &b10, c64 &b11) {
+        // B M_b: col0 = c B0 - s e^{-i phi} B1, col1 = s B0 + c e^{-i phi} B1
+        const c64 eb = {gb.er, -gb.ei};
+        const c64 x0 = cmul(eb, b01), x1 = cmul(eb, b11);
+        const c64 n00 = {gb.c * b00.re - gb.s * x0.re, gb.c * b00.im - gb.s * x0.im};
+        const c64 n01 = {gb.s * b00.re + gb.c * x0.re, gb.s * b00.im + gb.c * x0.im};
+        const c64 n10 = {gb.c * b10.re - gb.s * x1.re, gb.c * b10.im - gb.s * x1.im};
+        const c64 n11 = {gb.s * b10.re + gb.c * x1.re, gb.s * b10.im + gb.c * x1.im};
+        // M_a^H X: row0 = c X0 - s e^{i phi} X1, row1 = s X0 + c e^{i phi} X1
+        const c64 ea = {ga.er, ga.ei};
+        const c64 y0 = cmul(ea, n10), y1 = cmul(ea, n11);
+        b00 = {ga.c * n00.re - ga.s * y0.re, ga.c * n00.im - ga.s * y0.im};
+        b01 = {ga.c * n01.re - ga.s * y1.re, ga.c * n01.im - ga.s * y1.im};
+        b10 = {ga.s * n00.re + ga.c * y0.re, ga.s * n00.im + ga.c * y0.im};
+        b11 = {ga.s * n01.re + ga.c * y1.re, ga.s * n01.im + ga.c * y1.im};
+    };
     int sweep = 0;
     for (; sweep < max_sweeps && r > 1; ++sweep) {
         double off = 0.0, tot = 0.0;
@@ -144,43 +168,30 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
                 rp[tid] = (int16_t)p;
                 rq[tid] = (int16_t)q;
                 lg[((int64_t)sweep * nrounds + t) * npairs + tid] = g;
+                // the pair's own diagonal block (p,p), (p,q), (q,q): no other
+                // thread touches it this round
+                if (g.s != 0.0) {
+                    c64 b00 = S[packed(rr, p, p)], b01 = S[packed(rr, p, q)], b10 = cconj(b01),
+                        b11 = S[packed(rr, q, q)];
+                    rotate(g, g, b00, b01, b10, b11);
+                    S[packed(rr, p, p)] = {b00.re, 0.0};
+                    S[packed(rr, p, q)] = b01;
+                    S[packed(rr, q, q)] = {b11.re, 0.0};
+                }
             }
             __syncthreads();
+            // the off-diagonal pair blocks, each read and written once
             for (int e = tid; e < nblocks; e += nthr) {
                 const int a = blk_a[e], b = blk_b[e];
                 const Rot ga = rot[a], gb = rot[b];
                 if (ga.s == 0.0 && gb.s == 0.0) continue;
                 const int pa = rp[a], qa = rq[a], pb = rp[b], qb = rq[b];
-                // B = [[A(pa,pb), A(pa,qb)], [A(qa,pb), A(qa,qb)]]
                 c64 b00 = at(pa, pb), b01 = at(pa, qb), b10 = at(qa, pb), b11 = at(qa, qb);
-                // B M_b: col0 = c B0 - s e^{-i phi} B1, col1 = s B0 + c e^{-i phi} B1
-                const c64 eb = {gb.er, -gb.ei};
-                c64 x0 = cmul(eb, b01), x1 = cmul(eb, b11);
-                c64 n00 = {gb.c * b00.re - gb.s * x0.re, gb.c * b00.im - gb.s * x0.im};
-                c64 n01 = {gb.s * b00.re + gb.c * x0.re, gb.s * b00.im + gb.c * x0.im};
-                c64 n10 = {gb.c * b10.re - gb.s * x1.re, gb.c * b10.im - gb.s * x1.im};
-                c64 n11 = {gb.s * b10.re + gb.c * x1.re, gb.s * b10.im + gb.c * x1.im};
-                // M_a^H X: row0 = c X0 - s e^{i phi} X1, row1 = s X0 + c e^{i phi} X1
-                const c64 ea = {ga.er, ga.ei};
-                const c64 y0 = cmul(ea, n10), y1 = cmul(ea, n11);
-                b00 = {ga.c * n00.re - ga.s * y0.re, ga.c * n00.im - ga.s * y0.im};
-                b01 = {ga.c * n01.re - ga.s * y1.re, ga.c * n01.im - ga.s * y1.im};
-                b10 = {ga.s * n00.re + ga.c * y0.re, ga.s * n00.im + ga.c * y0.im};
-                b11 = {ga.s * n01.re + ga.c * y1.re, ga.s * n01.im + ga.c * y1.im};
-                if (a == b) {  // the diagonal block: (p,p), (p,q), (q,q)
-                    S[packed(rr, pa, pa)] = {b00.re, 0.0};
-                    S[packed(rr, pa, qa)] = b01;
-                    S[packed(rr, qa, qa)] = {b11.re, 0.0};
-                } else {
-                    auto put = [&](int i, int j, c64 v) {
-                        if (i <= j) S[packed(rr, i, j)] = v;
-                        else S[packed(rr, j, i)] = cconj(v);
-                    };
-                    put(pa, pb, b00);
-                    put(pa, qb, b01);
-                    put(qa, pb, b10);
-                    put(qa, qb, b11);
-                }
+                rotate(ga, gb, b00, b01, b10, b11);
+                put(pa, pb, b00);
+                put(pa, qb, b01);
+                put(qa, pb, b10);
+                put(qa, qb, b11);
             }
             __syncthreads();
         }
@@ -200,52 +211,96 @@ __global__ void __launch_bounds__(1024) k_jacobi_rotate(const c64 *__restrict__ 
     }
 }
 
-constexpr int EIGH_ROWS = 8;    // rows of V per workgroup
-constexpr int EIGH_BATCH = 8;   // logged rounds staged per batch
+constexpr int EIGH_ROWS = 4;   // rows of V per workgroup: one per wave
+constexpr int EIGH_BATCH = 8;  // logged rounds staged per batch (of 128 pairs at most)
 
 // Vout[k][row][rank[j]] = V[row][j] for rows/columns < r, V = product of the
-// logged rotations applied to the identity.
+// logged rotations applied to the identity.  Rows of V are independent under
+// column rotations: wave w rebuilds row 4 * blockIdx.x + w in LDS, lane l
+// applying the rotations of pairs l and l + 64 of every round.  The pairs of
+// a round are disjoint, and the rounds of a row run in order within its wave
+// (LDS operations of one wave complete in issue order), so the workgroup
+// synchronises once per batch of logged rounds.  Batches alternate between
+// two LDS buffers (two objects, so that the compiler sees that reads of one
+// never wait for the loads into the other): the next batch moves from memory
+// straight into the other buffer while one is applied.
 __global__ void __launch_bounds__(256) k_jacobi_vectors(c64 *__restrict__ Vout_all, int ld, int64_t stride,
                                                         const int32_t *__restrict__ rk,
                                                         const Rot *__restrict__ log, int64_t log_stride,
                                                         const int32_t *__restrict__ rounds_all,
                                                         const int32_t *__restrict__ rank_all) {
     __shared__ c64 V[EIGH_ROWS][EIGH_MAX];
-    __shared__ Rot R[EIGH_BATCH][EIGH_MAX / 2];
-    const int k = blockIdx.y, tid = threadIdx.x;
+    __shared__ __attribute__((aligned(1024))) Rot R0[EIGH_BATCH * (EIGH_MAX / 2)];  // u * npairs + a
+    __shared__ __attribute__((aligned(1024))) Rot R1[EIGH_BATCH * (EIGH_MAX / 2)];
+    const int k = blockIdx.y, tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int r = rk[k];
     const int row0 = blockIdx.x * EIGH_ROWS;
     if (row0 >= r) return;
     const int rr = r + (r & 1), npairs = rr / 2, nrounds = rr - 1;
     const int rounds = rounds_all[k];
     const Rot *lg = log + (int64_t)k * log_stride;
+    const int64_t log_bytes = (int64_t)rounds * npairs * (int64_t)sizeof(Rot);
+    const int batch_bytes = EIGH_BATCH * npairs * (int)sizeof(Rot);
+    // batch b of the log into `dst`, 1 KiB per wave instruction; a buffer
+    // load past the log's end returns zeros (those entries are never read)
+    auto fetch = [&](int b, Rot *dst) {
+        const int64_t off = (int64_t)b * batch_bytes;
+        const int bytes = (int)min<int64_t>(batch_bytes, log_bytes - off);
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char *>(reinterpret_cast<const char *>(lg) + off), 0, bytes, 0x00020000);
+        for (int c = wave; c * 1024 < bytes; c += EIGH_ROWS)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, reinterpret_cast<char *>(dst) + c * 1024, 16, lane * 16,
+                                                     c * 1024, 0, 0);
+    };
+    c64 *v = V[wave];
+    int t = 0;  // round within the sweep
+    auto apply = [&](int b, const Rot *Rb) {
+        const int nb = min(EIGH_BATCH, rounds - b * EIGH_BATCH);
+        for (int u = 0; u < nb; ++u) {
+            for (int a = lane; a < npairs; a += 64) {
+                int p, q;
+                rr_pair(rr, t, a, p, q);
+                // the rotation and both elements are read together and written
+                // back unconditionally (unchanged under an identity rotation):
+                // one LDS round trip per round
+                const Rot g = Rb[u * npairs + a];
+                const c64 xp = v[p], vq = v[q];
+                const c64 yq = cmul({g.er, -g.ei}, vq);
+                const bool rot = g.s != 0.0;
+                const double pr = g.c * xp.re - g.s * yq.re, pi = g.c * xp.im - g.s * yq.im;
+                const double qr = g.s * xp.re + g.c * yq.re, qi = g.s * xp.im + g.c * yq.im;
+                v[p] = {rot ? pr : xp.re, rot ? pi : xp.im};
+                v[q] = {rot ? qr : vq.re, rot ? qi : vq.im};
+            }
+            // this round's writes land before any lane of the wave reads for the next
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            t = t + 1 == nrounds ? 0 : t + 1;
+        }
+    };
+    // every wave's loads of a batch have landed (each waits for its own)
+    // before the barrier that opens the batch
+    constexpr int WAIT_VMEM = 0x0F70;  // s_waitcnt vmcnt(0)
     for (int e = tid; e < EIGH_ROWS * rr; e += 256) {
         const int lr = e / rr, j = e % rr;
         V[lr][j] = {row0 + lr == j ? 1.0 : 0.0, 0.0};
     }
-    const int items = EIGH_ROWS * npairs;
-    for (int t0 = 0; t0 < rounds; t0 += EIGH_BATCH) {
-        const int nb = min(EIGH_BATCH, rounds - t0);
-        __syncthreads();  // the previous batch is applied
-        for (int e = tid; e < nb * npairs; e += 256) R[e / npairs][e % npairs] = lg[(int64_t)t0 * npairs + e];
-        __syncthreads();
-        for (int u = 0; u < nb; ++u) {
-            const int t = (t0 + u) % nrounds;
-            for (int e = tid; e < items; e += 256) {
-                const int lr = e / npairs, a = e % npairs;
-                const Rot g = R[u][a];
-                if (g.s == 0.0) continue;
-                int p, q;
-                rr_pair(rr, t, a, p, q);
-                const c64 em = {g.er, -g.ei};
-                const c64 x = V[lr][p], y = cmul(em, V[lr][q]);
-                V[lr][p] = {g.c * x.re - g.s * y.re, g.c * x.im - g.s * y.im};
-                V[lr][q] = {g.s * x.re + g.c * y.re, g.s * x.im + g.c * y.im};
-            }
-            __syncthreads();
-        }
-    }
+    const int nbatch = (rounds + EIGH_BATCH - 1) / EIGH_BATCH;
+    if (nbatch > 0) fetch(0, R0);
+    __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
     __syncthreads();
+    for (int b = 0; b < nbatch; b += 2) {
+        if (b + 1 < nbatch) fetch(b + 1, R1);  // R1 was last read in batch b - 1
+        apply(b, R0);
+        __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
+        __syncthreads();
+        if (b + 1 >= nbatch) break;
+        if (b + 2 < nbatch) fetch(b + 2, R0);
+        apply(b + 1, R1);
+        __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
+        __syncthreads();
+    }
     c64 *Vout = Vout_all + k * stride;
     const int32_t *rank = rank_all + (int64_t)k * EIGH_MAX;
     for (int e = tid; e < EIGH_ROWS * r; e += 256) {
