@@ -172,9 +172,13 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     // adjacent lanes:
     //   X(i,c) = -sum_{j=c}^{i-1} L(i,j) X(j,c),           X(c,c) = 1
     //   Y(r,j) = (delta_rj - sum_{i=r}^{j-1} Y(r,i) U(i,j)) / U(j,j)
-    const int col = tid >> 2, part = tid & 3;
+    // Column c of X and row r of Y depend on nothing but themselves and the
+    // factored tile, and the four lanes of a column (row) sit in one wave, so
+    // the recurrences need no workgroup barrier: LDS operations of one wave
+    // complete in issue order.  Iteration i takes row i of X and column i of Y.
+    const int col = tid >> 2, part = tid & 3, row = col;
     for (int i = 0; i < TILE; ++i) {
-        float sr = 0.f, si = 0.f;
+        float sr = 0.f, si = 0.f, tr = 0.f, ti2 = 0.f;
         if (i > col) {
             for (int j = col + part; j < i; j += 4) {
                 const float ar = Are[i * LDA + j], ai = Aim[i * LDA + j];
@@ -183,45 +187,43 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
                 si += ar * xi + ai * xr;
             }
         }
+        const int jj = i;  // column of Y
+        if (jj > row) {
+            for (int q = row + part; q < jj; q += 4) {
+                const float yr = Ure[row * LDA + q], yi = Uim[row * LDA + q];
+                const float ur = Are[q * LDA + jj], ui = Aim[q * LDA + jj];
+                tr += yr * ur - yi * ui;
+                ti2 += yr * ui + yi * ur;
+            }
+        }
         sr += __shfl_xor(sr, 1);
         si += __shfl_xor(si, 1);
+        tr += __shfl_xor(tr, 1);
+        ti2 += __shfl_xor(ti2, 1);
         sr += __shfl_xor(sr, 2);
         si += __shfl_xor(si, 2);
+        tr += __shfl_xor(tr, 2);
+        ti2 += __shfl_xor(ti2, 2);
         if (part == 0) {
             Lre[i * LDA + col] = i < col ? 0.f : (i == col ? 1.f : -sr);
             Lim[i * LDA + col] = i <= col ? 0.f : -si;
-        }
-        __syncthreads();
-    }
-    const int row = col;
-    for (int j = 0; j < TILE; ++j) {
-        float sr = 0.f, si = 0.f;
-        if (j > row) {
-            for (int i = row + part; i < j; i += 4) {
-                const float yr = Ure[row * LDA + i], yi = Uim[row * LDA + i];
-                const float ur = Are[i * LDA + j], ui = Aim[i * LDA + j];
-                sr += yr * ur - yi * ui;
-                si += yr * ui + yi * ur;
-            }
-        }
-        sr += __shfl_xor(sr, 1);
-        si += __shfl_xor(si, 1);
-        sr += __shfl_xor(sr, 2);
-        si += __shfl_xor(si, 2);
-        if (part == 0) {
-            if (j < row) {
-                Ure[row * LDA + j] = 0.f;
-                Uim[row * LDA + j] = 0.f;
+            if (jj < row) {
+                Ure[row * LDA + jj] = 0.f;
+                Uim[row * LDA + jj] = 0.f;
             } else {
-                const float nr = (j == row ? 1.f : 0.f) - sr, ni = -si;
-                const float dr = Are[j * LDA + j], di = Aim[j * LDA + j];
+                const float nr = (jj == row ? 1.f : 0.f) - tr, ni = -ti2;
+                const float dr = Are[jj * LDA + jj], di = Aim[jj * LDA + jj];
                 const float dd = dr * dr + di * di;
-                Ure[row * LDA + j] = (nr * dr + ni * di) / dd;
-                Uim[row * LDA + j] = (ni * dr - nr * di) / dd;
+                Ure[row * LDA + jj] = (nr * dr + ni * di) / dd;
+                Uim[row * LDA + jj] = (ni * dr - nr * di) / dd;
             }
         }
-        __syncthreads();
+        // this iteration's writes land before the wave's reads of the next
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    __syncthreads();  // X and Y are complete before the copy-out below
     float *dv = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
     for (int e = tid; e < TILE2; e += 256) {
         const int r = e / TILE, c = e % TILE;

@@ -510,6 +510,16 @@ void Bundle::plan_schur(int group) {
         fwd_row_ptr.push_back((int64_t)frows.size());
         bwd_row_ptr.push_back((int64_t)brows.size());
     }
+    if (std::getenv("AOSX_DEBUG")) {
+        // trailing solve work: rows (one block each per matrix and chunk
+        // group) and the tiles they take (their steps)
+        int64_t tiles_below = 0;
+        for (size_t g = 0; g + 1 < grp_begin.size(); ++g)
+            for (int k = grp_begin[g]; k < grp_begin[g + 1]; ++k)
+                for (int32_t a = ts.next_ptr[k]; a < ts.next_ptr[k + 1]; ++a) tiles_below += ts.next_idx[a] >= grp_begin[g + 1];
+        std::fprintf(stderr, "AOSX_DEBUG|solve_plan|fwd_rows=%zu|tiles=%lld|steps_per_row=%.2f\n", frows.size(),
+                     (long long)tiles_below, frows.empty() ? 0.0 : (double)tiles_below / (double)frows.size());
+    }
     // the launch sequences captured so far follow the old plan
     DeviceLock lock(g_device_lock);
     clear_graphs();
