@@ -105,10 +105,13 @@ struct Bundle {
     // resident device data
     float2 *d_hval = nullptr, *d_sval = nullptr;
     int32_t *d_source = nullptr, *d_group = nullptr, *d_starts = nullptr, *d_mirror = nullptr;
-    int32_t *d_rows = nullptr, *d_indices = nullptr, *d_indptr = nullptr, *d_shifts = nullptr;
+    int32_t *d_indices = nullptr, *d_indptr = nullptr, *d_shifts = nullptr;
     int32_t *d_iperm = nullptr, *d_perm = nullptr, *d_tile_id = nullptr, *d_next = nullptr, *d_prev = nullptr;
+    // CSR entries of each stored tile (sources, positions; source -1: a padding
+    // row's unit diagonal), for k_fill_tiles
+    int32_t *d_tptr = nullptr, *d_tsrc = nullptr;
+    uint16_t *d_tpos = nullptr;
     int32_t *d_atom_off = nullptr;   // [n_atoms + 1] orbital offsets (atom-block SpMM)
-    int32_t *d_padding = nullptr;    // factor rows that hold no orbital (unit diagonal)
     // Step groups of the factorisation: steps [grp_begin[g], grp_begin[g+1])
     // update the tiles of their own rows and columns step by step; all other
     // targets take the group's updates in one trailing launch over the pair

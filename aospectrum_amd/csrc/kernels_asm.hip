@@ -42,45 +42,67 @@ __global__ void k_phases_byval(float2 *__restrict__ phase, const int32_t *__rest
     phase[e] = bloch_phase(shifts + 3 * g, kpts.k[k]);
 }
 
+// acc += v * p (complex64)
+__device__ __forceinline__ void bloch_term(float2 &acc, float2 v, float2 p) {
+    acc.x += v.x * p.x - v.y * p.y;
+    acc.y += v.x * p.y + v.y * p.x;
+}
 __device__ __forceinline__ float2 bloch_sum(const float2 *__restrict__ vals, const int32_t *__restrict__ source,
                                             const int32_t *__restrict__ group, const float2 *__restrict__ ph,
                                             int32_t s0, int32_t s1) {
     float2 acc = make_float2(0.f, 0.f);
-    for (int32_t q = s0; q < s1; ++q) {
-        const float2 v = vals[source[q]], p = ph[group[q]];
-        acc.x += v.x * p.x - v.y * p.y;
-        acc.y += v.x * p.y + v.y * p.x;
-    }
+    for (int32_t q = s0; q < s1; ++q) bloch_term(acc, vals[source[q]], ph[group[q]]);
     return acc;
 }
+// (v_p + conj(v_q)) / 2, the real part on the diagonal
+__device__ __forceinline__ float2 hermitian_mean(float2 v0, float2 v1, bool diag) {
+    return diag ? make_float2(v0.x, 0.f) : make_float2((v0.x + v1.x) * 0.5f, (v0.y - v1.y) * 0.5f);
+}
 
-// out_h[k][p], out_s[k][p] for nk k points (grid.y = k)
+// out_h[k][p], out_s[k][p] for nk k points.  A thread takes entry p for every
+// k point: it reads the gather lists and source values of p and of its
+// mirror once and forms the nk phase sums from the phase table (nk x groups,
+// cache-resident), so that the lists and the gathered values cross HBM once
+// per launch instead of once per k point.
 __global__ void __launch_bounds__(256) k_assemble(float2 *__restrict__ out_h, float2 *__restrict__ out_s,
                                                  const float2 *__restrict__ hv, const float2 *__restrict__ sv,
                                                  const int32_t *__restrict__ source,
                                                  const int32_t *__restrict__ group,
                                                  const int32_t *__restrict__ starts,
                                                  const int32_t *__restrict__ mirror,
-                                                 const float2 *__restrict__ phase, int n_groups, int64_t nnz) {
-    const int k = blockIdx.y;
-    const float2 *ph = phase + (int64_t)k * n_groups;
+                                                 const float2 *__restrict__ phase, int n_groups, int64_t nnz, int nk) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
         const int32_t q = mirror[p];
-        const float2 h0 = bloch_sum(hv, source, group, ph, starts[p], starts[p + 1]);
-        const float2 s0 = bloch_sum(sv, source, group, ph, starts[p], starts[p + 1]);
-        float2 hres, sres;
-        if (q == p) {
-            hres = make_float2(h0.x, 0.f);
-            sres = make_float2(s0.x, 0.f);
+        const bool diag = q == p;
+        const int32_t a0 = starts[p], a1 = starts[p + 1];
+        const int32_t b0 = diag ? a0 : starts[q], b1 = diag ? a1 : starts[q + 1];
+        if (a1 - a0 == 1 && b1 - b0 == 1) {
+            // one source entry each (no duplicate lattice shifts): values and
+            // groups in registers across the k points
+            const int32_t sa = source[a0], sb = source[b0], ga = group[a0], gb = group[b0];
+            const float2 ha = hv[sa], xa = sv[sa], hb = hv[sb], xb = sv[sb];
+            for (int k = 0; k < nk; ++k) {
+                const float2 pa = phase[(int64_t)k * n_groups + ga], pb = phase[(int64_t)k * n_groups + gb];
+                float2 h0 = make_float2(0.f, 0.f), s0 = h0, h1 = h0, s1 = h0;
+                bloch_term(h0, ha, pa);
+                bloch_term(s0, xa, pa);
+                bloch_term(h1, hb, pb);
+                bloch_term(s1, xb, pb);
+                out_h[(int64_t)k * nnz + p] = hermitian_mean(h0, h1, diag);
+                out_s[(int64_t)k * nnz + p] = hermitian_mean(s0, s1, diag);
+            }
         } else {
-            const float2 h1 = bloch_sum(hv, source, group, ph, starts[q], starts[q + 1]);
-            const float2 s1 = bloch_sum(sv, source, group, ph, starts[q], starts[q + 1]);
-            hres = make_float2((h0.x + h1.x) * 0.5f, (h0.y - h1.y) * 0.5f);
-            sres = make_float2((s0.x + s1.x) * 0.5f, (s0.y - s1.y) * 0.5f);
+            for (int k = 0; k < nk; ++k) {
+                const float2 *ph = phase + (int64_t)k * n_groups;
+                const float2 h0 = bloch_sum(hv, source, group, ph, a0, a1);
+                const float2 s0 = bloch_sum(sv, source, group, ph, a0, a1);
+                const float2 h1 = bloch_sum(hv, source, group, ph, b0, b1);
+                const float2 s1 = bloch_sum(sv, source, group, ph, b0, b1);
+                out_h[(int64_t)k * nnz + p] = hermitian_mean(h0, h1, diag);
+                out_s[(int64_t)k * nnz + p] = hermitian_mean(s0, s1, diag);
+            }
         }
-        out_h[(int64_t)k * nnz + p] = hres;
-        out_s[(int64_t)k * nnz + p] = sres;
     }
 }
 

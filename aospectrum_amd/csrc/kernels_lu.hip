@@ -38,38 +38,63 @@ __global__ void k_fill_zero(float4 *__restrict__ p, int64_t count4) {
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// A_m = H_k - z_m S_k scattered into the tile storage of matrix m
-// (matrix m uses k point kidx[m] and shift z[m]); padding rows get a unit
-// diagonal so their tiles stay nonsingular.
-__global__ void k_scatter_shifted(float *__restrict__ tiles, int64_t mat_stride, const float2 *__restrict__ h,
-                                  const float2 *__restrict__ s, int64_t nnz, const int32_t *__restrict__ kidx,
-                                  const float2 *__restrict__ z, const int32_t *__restrict__ rows,
-                                  const int32_t *__restrict__ cols, const int32_t *__restrict__ iperm,
-                                  const int32_t *__restrict__ tile_id, int nt,
-                                  const int32_t *__restrict__ padding_rows, int n_padding) {
-    const int m = blockIdx.y;
-    const float2 zm = z[m];
-    const float2 *hk = h + (int64_t)kidx[m] * nnz;
-    const float2 *sk = s + (int64_t)kidx[m] * nnz;
-    float *base = tiles + (int64_t)m * mat_stride;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride) {
-        const float2 hv = hk[p], sv = sk[p];
-        const float re = hv.x - (zm.x * sv.x - zm.y * sv.y);
-        const float im = hv.y - (zm.x * sv.y + zm.y * sv.x);
-        const int32_t fr = iperm[rows[p]], fc = iperm[cols[p]];
-        const int32_t id = tile_id[(int64_t)(fr / TILE) * nt + fc / TILE];
-        if (id < 0) continue;  // cannot happen for a closed tile structure
-        float *t = base + (int64_t)id * TILE_FLOATS + (fr % TILE) * TILE + fc % TILE;
-        t[0] = re;
-        t[PLANE] = im;
+// A_m = H_k - z_m S_k (matrix m uses k point kidx[m] and shift z[m]) written
+// tile by tile: workgroup b holds stored tile b in LDS -- zeros, the tile's
+// CSR entries (source p, position pos = 64 r + c in the tile) and the unit
+// diagonals of padding rows (source -1), so their tiles stay nonsingular --
+// and writes it out whole for every matrix of the batch.  The factor storage
+// is thus written once, coalesced, with no separate zero fill; the H and S
+// values of an entry are loaded once per k point (the matrices of one k point
+// are consecutive).
+__global__ void __launch_bounds__(256) k_fill_tiles(float *__restrict__ tiles, int64_t mat_stride,
+                                                   const float2 *__restrict__ h, const float2 *__restrict__ s,
+                                                   int64_t nnz, const int32_t *__restrict__ kidx,
+                                                   const float2 *__restrict__ z, int nmat,
+                                                   const int32_t *__restrict__ tptr, const int32_t *__restrict__ tsrc,
+                                                   const uint16_t *__restrict__ tpos) {
+    __shared__ float4 T[TILE_FLOATS / 4];  // re plane, im plane (32 KiB)
+    constexpr int EPT = TILE2 / 256;       // entries per thread (a tile holds at most 64 x 64)
+    float *Tf = reinterpret_cast<float *>(T);
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    for (int i = tid; i < TILE_FLOATS / 4; i += 256) T[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int e0 = tptr[b], ne = tptr[b + 1] - e0;
+    int32_t src[EPT];
+    int pos[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const int e = tid + 256 * j;
+        src[j] = e < ne ? tsrc[e0 + e] : -2;
+        pos[j] = e < ne ? (int)tpos[e0 + e] : 0;
     }
-    if (blockIdx.x == 0) {
-        for (int q = threadIdx.x; q < n_padding; q += blockDim.x) {
-            const int32_t r = padding_rows[q];
-            const int32_t id = tile_id[(int64_t)(r / TILE) * nt + r / TILE];
-            base[(int64_t)id * TILE_FLOATS + (r % TILE) * TILE + r % TILE] = 1.0f;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+        if (src[j] == -1) Tf[pos[j]] = 1.0f;
+    float2 hv[EPT], sv[EPT];
+    int kprev = -1;
+    for (int m = 0; m < nmat; ++m) {
+        const int k = kidx[m];
+        const float2 zm = z[m];
+        if (k != kprev) {
+            const float2 *hk = h + (int64_t)k * nnz, *sk = s + (int64_t)k * nnz;
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) {
+                hv[j] = src[j] >= 0 ? hk[src[j]] : make_float2(0.f, 0.f);
+                sv[j] = src[j] >= 0 ? sk[src[j]] : make_float2(0.f, 0.f);
+            }
+            kprev = k;
         }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j)
+            if (src[j] >= 0) {
+                Tf[pos[j]] = hv[j].x - (zm.x * sv[j].x - zm.y * sv[j].y);
+                Tf[PLANE + pos[j]] = hv[j].y - (zm.x * sv[j].y + zm.y * sv[j].x);
+            }
+        __syncthreads();
+        float4 *dst = reinterpret_cast<float4 *>(tiles + (int64_t)m * mat_stride + b * TILE_FLOATS);
+        for (int i = tid; i < TILE_FLOATS / 4; i += 256) dst[i] = T[i];
+        __syncthreads();  // the tile is read out before the next matrix's values go in
     }
 }
 

@@ -218,11 +218,11 @@ Bundle::~Bundle() {
         clear_graphs();
         release_workspace();
         for (void *p : {(void *)d_hval, (void *)d_sval, (void *)d_source, (void *)d_group, (void *)d_starts,
-                        (void *)d_mirror, (void *)d_rows, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
+                        (void *)d_mirror, (void *)d_indices, (void *)d_indptr, (void *)d_shifts,
                         (void *)d_iperm, (void *)d_perm, (void *)d_tile_id, (void *)d_next, (void *)d_prev,
                         (void *)d_atom_off, (void *)d_pairs, (void *)d_quads, (void *)d_ppos, (void *)d_step_s,
                         (void *)d_fwd_rows, (void *)d_bwd_rows, (void *)d_pair_mask, (void *)d_quad_mask,
-                        (void *)d_padding})
+                        (void *)d_tptr, (void *)d_tsrc, (void *)d_tpos})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -310,14 +310,42 @@ void Bundle::upload(const void *h_values, const void *s_values, int value_kind, 
     d_starts = upload_vec(starts32, resident_bytes);
     d_group = upload_vec(pat.group, resident_bytes);
     d_mirror = upload_vec(pat.mirror, resident_bytes);
-    d_rows = upload_vec(pat.rows, resident_bytes);
     d_indices = upload_vec(pat.indices, resident_bytes);
     d_indptr = upload_vec(pat.indptr, resident_bytes);
     d_shifts = upload_vec(pat.shifts, resident_bytes);
     d_iperm = upload_vec(ts.iperm, resident_bytes);
     d_perm = upload_vec(ts.perm, resident_bytes);
-    d_padding = upload_vec(ts.padding_rows, resident_bytes);
     d_tile_id = upload_vec(ts.tile_id, resident_bytes);
+    {
+        // entries of every stored tile by counting sort on the tile id
+        const int nt = ts.n_diag;
+        const int64_t npad = (int64_t)ts.padding_rows.size();
+        auto tile_of = [&](int32_t fr, int32_t fc) {
+            const int32_t id = ts.tile_id[(int64_t)(fr / TILE) * nt + fc / TILE];
+            if (id < 0) throw Error(AOSX_INVALID_BLOCKS, "pattern entry outside the tile structure");
+            return id;
+        };
+        std::vector<int32_t> tptr(ts.n_tiles + 1, 0), tsrc(nnz + npad);
+        std::vector<uint16_t> tpos(nnz + npad);
+        for (int64_t p = 0; p < nnz; ++p) ++tptr[tile_of(ts.iperm[pat.rows[p]], ts.iperm[pat.indices[p]]) + 1];
+        for (int32_t r : ts.padding_rows) ++tptr[tile_of(r, r) + 1];
+        for (int64_t t = 0; t < ts.n_tiles; ++t) tptr[t + 1] += tptr[t];
+        std::vector<int32_t> cur(tptr.begin(), tptr.end() - 1);
+        for (int64_t p = 0; p < nnz; ++p) {
+            const int32_t fr = ts.iperm[pat.rows[p]], fc = ts.iperm[pat.indices[p]];
+            const int32_t e = cur[tile_of(fr, fc)]++;
+            tsrc[e] = (int32_t)p;
+            tpos[e] = (uint16_t)((fr % TILE) * TILE + fc % TILE);
+        }
+        for (int32_t r : ts.padding_rows) {
+            const int32_t e = cur[tile_of(r, r)]++;
+            tsrc[e] = -1;
+            tpos[e] = (uint16_t)((r % TILE) * TILE + r % TILE);
+        }
+        d_tptr = upload_vec(tptr, resident_bytes);
+        d_tsrc = upload_vec(tsrc, resident_bytes);
+        d_tpos = upload_vec(tpos, resident_bytes);
+    }
     d_next = upload_vec(ts.next_idx, resident_bytes);
     d_prev = upload_vec(ts.prev_idx, resident_bytes);
     n_atoms = atoms;
@@ -618,17 +646,18 @@ void Bundle::assemble(const double *kpts_host, int64_t nk, float2 *d_h, float2 *
     k_phases<<<blocks_for(ng * nk, 256, 1 << 20), 256, 0, stream>>>(d_ph, d_shifts, ng, d_k, (int)nk);
     AOSX_LAUNCHED();
     EventTimer tm(kstats.enabled, stream);
-    dim3 grid(blocks_for(nnz, 256, 4096), (unsigned)nk);
-    k_assemble<<<grid, 256, 0, stream>>>(d_h, d_s, d_hval, d_sval, d_source, d_group, d_starts, d_mirror, d_ph,
-                                         ng, nnz);
+    k_assemble<<<blocks_for(nnz, 256, 8192), 256, 0, stream>>>(d_h, d_s, d_hval, d_sval, d_source, d_group,
+                                                                d_starts, d_mirror, d_ph, ng, nnz, (int)nk);
     AOSX_LAUNCHED();
     if (kstats.enabled) {
         kstats.assemble_ms += tm.stop();
         kstats.assemble_launches += 1;
-        // algorithmic bytes: each CSR entry gathers its own and its mirror's
-        // source values of H and S and writes two complex64 values
+        // algorithmic bytes: each source entry's H and S values, source and
+        // group index are read for its own CSR entry and for the mirror's;
+        // each CSR entry reads its mirror and start offset and writes H and S
+        // (complex64) for every k point
         const double src = (double)pat.source.size();
-        kstats.assemble_bytes += nk * (2.0 * 2.0 * src * (8.0 + 4.0 + 4.0) + nnz * (2 * 8.0 + 4 * 4.0 + 4.0));
+        kstats.assemble_bytes += 2.0 * src * (2 * 8.0 + 4.0 + 4.0) + nnz * (4.0 + 4.0) + nk * nnz * 2 * 8.0;
     }
 }
 
@@ -645,8 +674,8 @@ void Bundle::assemble_noalloc(const double *kpts_host, int nk, float2 *d_h, floa
     const int ng = (int)n_groups;
     k_phases_byval<<<blocks_for((int64_t)ng * nk, 256, 1 << 20), 256, 0, stream>>>(phase, d_shifts, ng, kb, nk);
     AOSX_LAUNCHED();
-    k_assemble<<<dim3(blocks_for(nnz, 256, 4096), (unsigned)nk), 256, 0, stream>>>(
-        d_h, d_s, d_hval, d_sval, d_source, d_group, d_starts, d_mirror, phase, ng, nnz);
+    k_assemble<<<blocks_for(nnz, 256, 8192), 256, 0, stream>>>(d_h, d_s, d_hval, d_sval, d_source, d_group,
+                                                                d_starts, d_mirror, phase, ng, nnz, nk);
     AOSX_LAUNCHED();
     AOSX_HIP(hipStreamSynchronize(stream));
 }
@@ -711,12 +740,8 @@ void Bundle::factor(int nmat, const std::vector<int32_t> &kidx, const std::vecto
         }
     };
     auto launches = [&]() {
-        const int64_t count4 = (int64_t)nmat * mstride / 4;
-        k_fill_zero<<<blocks_for(count4, 256, 8192), 256, 0, stream>>>(reinterpret_cast<float4 *>(tiles), count4);
-        AOSX_LAUNCHED();
-        k_scatter_shifted<<<dim3(blocks_for(nnz, 256, 4096), nmat), 256, 0, stream>>>(
-            tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, d_rows, d_indices, d_iperm, d_tile_id, ts.n_diag, d_padding,
-            (int)ts.padding_rows.size());
+        k_fill_tiles<<<(unsigned)ts.n_tiles, 256, 0, stream>>>(tiles, mstride, d_h, d_s, nnz, d_kidx, d_z, nmat,
+                                                                d_tptr, d_tsrc, d_tpos);
         AOSX_LAUNCHED();
         for (int g = 0; g < ng; ++g) {
             const int g0 = grp_begin[g], g1 = grp_begin[g + 1];
