@@ -145,6 +145,9 @@ class BandServer:
         self._closed = False
         self._next_id = 0
         self.batch_log: list[dict] = []
+        # bundles whose next batch waits for memory: (arrival of its oldest
+        # k point, bytes it needs)
+        self._waiting: dict[str, tuple[float, int]] = {}
 
     # ----------------------------------------------------------- bundles
     def register(self, key: str, bundle: LocalizedOperatorBundle, *, autotune: CissSettings | None = None) -> dict:
@@ -229,6 +232,19 @@ class BandServer:
         res.pending = keep
         return take
 
+    def _admit(self, res: _Resident, need: int, arrival: float) -> bool:
+        """Hold the lock.  Admission in order of arrival across bundles: a
+        batch whose oldest k point arrived after that of another bundle's
+        waiting batch starts only if both fit beside the running batches
+        (idle caches can be freed), so that a bundle which finishes a batch
+        and takes the next one at once cannot starve the others."""
+        older = sum(n for k, (a, n) in self._waiting.items() if k != res.key and a < arrival)
+        if older:
+            running = sum(r.reserved for r in self._bundles.values() if r is not res)
+            if running + need + older > self.budget:
+                return False
+        return self._reserve(res, need)
+
     def _reserve(self, res: _Resident, need: int) -> bool:
         """Hold the lock; make room for `need` bytes or return False.
 
@@ -279,8 +295,11 @@ class BandServer:
                 while len(batch) > 1 and self.estimate_bytes(res.key, settings, len(batch)) > self.budget:
                     res.pending.appendleft(batch.pop())
                 need = self.estimate_bytes(res.key, settings, len(batch))
-                while not self._reserve(res, need):
+                arrival = min(it.job.arrival for it in batch)
+                self._waiting[res.key] = (arrival, need)
+                while not self._admit(res, need, arrival):
                     self._lock.wait(timeout=0.05)
+                del self._waiting[res.key]
                 res.busy = True
             self._run(res, batch, settings)
             with self._lock:

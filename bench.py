@@ -431,7 +431,7 @@ def main() -> int:
             names = ["small2d", "stress2d", "bulk3d", "slab26k_a"]
             known = {args.workload: plain_info["plain_seconds_per_kpoint"]} if plain_info else {}
             per_k = plain_seconds_per_k(names, args, known)
-            serving = serving_leg(args, jobs=6, kscale=0.1, names=names, plain_loop=False, plain_per_k=per_k)
+            serving = serving_leg(args, jobs=10, kscale=0.1, names=names, plain_loop=False, plain_per_k=per_k)
         except Exception as exc:  # noqa: BLE001 -- the main figure stands on its own
             serving = {"error": f"{type(exc).__name__}: {exc}"[:300]}
         log(f"serving: {serving}")

@@ -75,3 +75,24 @@ def test_own_cache_larger_than_room_is_freed():
     # b keeps 30 reserved; a's new batch of 60 fits, its (freed) cache is 0
     assert server._reserve(a, 60)
     assert (a.reserved, b.reserved) == (60, 30)
+
+
+def test_admission_follows_arrival_across_bundles():
+    """A bundle that has just finished a batch cannot take the memory again
+    while an older batch of another bundle waits, unless both fit."""
+    server, a, b = _server(100)
+    assert server._reserve(a, 70)
+    a.busy = True
+    # b's batch (arrived at t=1) waits: it does not fit beside a's batch
+    server._waiting["b"] = (1.0, 60)
+    assert not server._admit(b, 60, 1.0)
+    _end_batch(server, a)
+    # a's next batch arrived later (t=2): it must not take the room b waits for
+    server._waiting["a"] = (2.0, 70)
+    assert not server._admit(a, 70, 2.0)
+    # b goes first (a's idle cache is freed for it)
+    assert server._admit(b, 60, 1.0)
+    del server._waiting["b"]
+    b.busy = True
+    # a small later batch may start beside b when it leaves room
+    assert server._admit(a, 30, 2.0)
