@@ -321,7 +321,8 @@ __device__ __forceinline__ void stage_panel_block(const TileBlock &blk, uint16_t
 // block, started at x1 = Re, x2 = 0, x3 = Re + Im).  P is split once per step
 // into LDS (six planes, 48 KiB) from the FP32 tile, which is loaded into
 // registers a step ahead; Y_k comes split, transposed and in MFMA block order
-// from its slot, straight from memory into registers, one K quarter ahead, so
+// from its slot, straight from memory into registers, one or two K quarters
+// ahead, so
 // a step takes two barriers.  NC = 2: wave w computes the 64 rows x 32 columns
 // of column half w % 2 of chunk c0 + w / 2 (two 32 x 32 blocks); NC = 1: the
 // 32 x 32 block (w / 2, w % 2) of chunk c0.
@@ -375,7 +376,11 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail_3m(float *__restrict__ y
     if (k < g1) tn = tile_block_load(tm + (int64_t)tile_of(k) * TILE_FLOATS, tid);
     // B operands of K quarter q of step kk: blocks (q, component, part, wb)
     // of the slot of Y_kk, chunk c
-    u32x4 bq[2][3][2];
+    // B loads run QA K quarters ahead through a ring of 2 QA register sets:
+    // two with NC = 2 (the registers fit beside two waves per SIMD), one with
+    // NC = 1 (which keeps three waves per SIMD)
+    constexpr int QA = NC == 2 ? 2 : 1;
+    u32x4 bq[2 * QA][3][2];
     auto fetch_b = [&](u32x4 (&dst)[3][2], int kk, int q) {
         const uint16_t *sl = yslot(ysplit, m, kk, nchunk, c) + lane * 8;
 #pragma unroll
@@ -384,7 +389,9 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail_3m(float *__restrict__ y
             for (int h = 0; h < 2; ++h)
                 dst[comp][h] = *reinterpret_cast<const u32x4 *>(sl + panel_block(q, comp, h, wb) * SPLIT_BLOCK);
     };
-    if (k < g1 && active) fetch_b(bq[0], k, 0);
+    if (k < g1 && active)
+#pragma unroll
+        for (int q = 0; q < QA; ++q) fetch_b(bq[q], k, q);
     while (k < g1) {
         const int kn = next_k(k + 1);
         lds_barrier();  // the previous step's reads of As are done
@@ -394,10 +401,11 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail_3m(float *__restrict__ y
         if (active) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                // the next quarter's B operands (the next step's first quarter
-                // after the last), loading while this quarter's products run
-                if (q < 3) fetch_b(bq[(q + 1) & 1], k, q + 1);
-                else if (kn < g1) fetch_b(bq[0], kn, 0);
+                // the B operands QA quarters on (the next step's first ones
+                // after this step's last), loading while the products of this
+                // quarter and the next QA - 1 run
+                if (q + QA < 4) fetch_b(bq[(q + QA) % (2 * QA)], k, q + QA);
+                else if (kn < g1) fetch_b(bq[(q + QA) % (2 * QA)], kn, q + QA - 4);
 #pragma unroll
                 for (int comp = 0; comp < 3; ++comp) {
                     bf16x8 ah[NB], al[NB];
@@ -406,8 +414,8 @@ __global__ void __launch_bounds__(256, 2) k_solve_trail_3m(float *__restrict__ y
                         ah[b] = lds8(As + panel_block(q, comp, 0, wa + b) * SPLIT_BLOCK + lane * 8);
                         al[b] = lds8(As + panel_block(q, comp, 1, wa + b) * SPLIT_BLOCK + lane * 8);
                     }
-                    const bf16x8 bh = __builtin_bit_cast(bf16x8, bq[q & 1][comp][0]);
-                    const bf16x8 bl = __builtin_bit_cast(bf16x8, bq[q & 1][comp][1]);
+                    const bf16x8 bh = __builtin_bit_cast(bf16x8, bq[q % (2 * QA)][comp][0]);
+                    const bf16x8 bl = __builtin_bit_cast(bf16x8, bq[q % (2 * QA)][comp][1]);
 #pragma unroll
                     for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
