@@ -473,6 +473,36 @@ def test_bulk3d_matches_golden_inertia_counts():
         assert r.maximum_residual_ev <= 1e-4
 
 
+def test_bulk3d_full_path_headline_settings_match_golden_counts():
+    """The benchmark's configuration (4 k points per batch, 16 conjugate
+    pairs, BF16x3 factors, refinement on demand, 2 filter passes) over all 121
+    path points of the 13,000-orbital bulk: every in-window count equals the
+    committed FP64 Sylvester-inertia count and every residual is <= 1e-4 eV."""
+    import json
+    from pathlib import Path
+
+    golden = json.loads((Path(__file__).parent / "golden" / "bulk3d_counts.json").read_text())
+    bundle = synth.generate(synth.SPECS["bulk3d"])
+    lo, hi = golden["interval_ev"]
+    kp = np.asarray(golden["kpoints"])
+    settings = CissSettings(contour_batch_size=16, k_batch=4, conjugate_pairs=True, precision="bf16x3_ir",
+                            refinement_steps=2, refinement_passes=2, refinement_policy="on_demand")
+    say("bulk3d full path, 121 k points")
+    nb = NativeBundle(bundle, device=0)
+    try:
+        got = []
+        for b0 in range(0, kp.shape[0], 20):
+            results, info = band_solve(nb, settings, EnergyInterval(lo, hi), kpoints=kp[b0:b0 + 20])
+            got += [(r.energies_ev.size, r.maximum_residual_ev) for r in results]
+            say(f"  k {b0}..{b0 + len(results) - 1}: counts {[c for c, _ in got[b0:]]} refined {info['refined']}")
+    finally:
+        nb.close()
+    counts = [c for c, _ in got]
+    mismatch = {i: (c, g) for i, (c, g) in enumerate(zip(counts, golden["counts"])) if c != g}
+    assert not mismatch, mismatch
+    assert max(r for _, r in got) <= 1e-4
+
+
 @pytest.fixture(scope="module")
 def stress2d():
     return synth.generate(synth.SPECS["stress2d"])
