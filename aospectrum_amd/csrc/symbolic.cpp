@@ -9,6 +9,7 @@
 #include "symbolic.h"
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <cstring>
 #include <functional>
@@ -251,7 +252,11 @@ std::string order_atoms(const BlockTable &bt, int ordering, std::vector<int32_t>
 // (median of the widest axis, down to pairs), so that the consecutive atoms a
 // tile holds are spatial neighbours: a separator plane is cut into compact
 // patches instead of strips along its normal, and the tiles of a node couple
-// to few tiles of the nodes around it.
+// to few tiles of the nodes around it.  A separator's atoms are then ordered
+// (stably) by the node of their earliest already-ordered neighbour, so that
+// the atoms a lower node couples to sit in few consecutive tiles of the
+// separator (tile-level factorisation flops: bulk3d 3.361e12 -> 3.295e12,
+// 52,000-orbital slab 3.327e13 -> 3.293e13).
 std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<int32_t> &order,
                            std::vector<int32_t> &groups) {
     const int64_t na = bt.n_atoms;
@@ -289,8 +294,23 @@ std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<
         bisect(b, mid);
         bisect(mid, e);
     };
-    auto emit = [&](std::vector<int32_t> &node) {
+    std::vector<int32_t> where(na, -1);  // elimination position of the atoms ordered so far
+    auto emit = [&](std::vector<int32_t> &node, bool separator) {
         bisect(node.data(), node.data() + node.size());
+        if (separator) {
+            std::vector<std::pair<int64_t, int32_t>> kv(node.size());
+            for (size_t q = 0; q < node.size(); ++q) {
+                int32_t first = -1;
+                for (int32_t nb : adj[node[q]])
+                    if (where[nb] >= 0 && (first < 0 || where[nb] < first)) first = where[nb];
+                const int64_t g = first < 0 ? INT64_MAX
+                                            : std::upper_bound(groups.begin(), groups.end(), first) - groups.begin();
+                kv[q] = {g, node[q]};
+            }
+            std::stable_sort(kv.begin(), kv.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+            for (size_t q = 0; q < node.size(); ++q) node[q] = kv[q].second;
+        }
+        for (size_t q = 0; q < node.size(); ++q) where[node[q]] = (int32_t)(order.size() + q);
         order.insert(order.end(), node.begin(), node.end());
         groups.push_back((int32_t)order.size());
     };
@@ -300,7 +320,7 @@ std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<
     // explicit recursion: emit(region) = emit(A'), emit(B'), separator
     std::function<void(std::vector<int32_t> &)> nd = [&](std::vector<int32_t> &set) {
         if (set.size() <= leaf) {
-            emit(set);
+            emit(set, false);
             return;
         }
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -346,12 +366,12 @@ std::string order_atoms_nd(const BlockTable &bt, const double *pos, std::vector<
         }
         for (int32_t a : sep) covered[a] = 0;
         if (A.empty() || B.empty() || sep.size() * 10 > set.size() * 9) {
-            emit(set);
+            emit(set, false);
             return;
         }
         nd(A);
         nd(B);
-        emit(sep);
+        emit(sep, true);
     };
     std::vector<int32_t> all(na);
     std::iota(all.begin(), all.end(), 0);
