@@ -45,15 +45,23 @@ __global__ void __launch_bounds__(256) k_spmm(float2 *__restrict__ y, const floa
 //   RESIDUAL: out[m] = b[b_index[m]] - (H_k - z_m S_k) x[m], with ||out||^2 and
 //             ||b||^2 added to sumsq[m], bsq[m] (FP64 iterative refinement);
 //   else    : out[m] = A_k x[x_index[m]] with A = h (H or S).
-// The 8 half-waves of the block split the column list (half-wave q takes
-// entries q, q+8, ...) and each accumulates every row of the atom (up to MAXR)
-// for 32 columns of x, so each gathered x element is loaded once per block and
-// feeds all of the atom's rows; the partial sums are reduced through LDS.
+// The 8 half-waves of the block take NCC = min(4, ceil(w / 32)) (rounded up
+// to a power of two) 32-column chunks of x at a time and split the column
+// list S = 8 / NCC ways per chunk (stream s of a chunk takes entries s, s + S,
+// ...); each accumulates every row of the atom (up to MAXR) for its 32
+// columns, so each gathered x element is loaded once per block and feeds all
+// of the atom's rows, and the values of a chunk of the column list are staged
+// once for all NCC column chunks.  The partial sums of a column chunk's
+// streams are added across the two half-waves of a wave by a shuffle and,
+// when a chunk spans several waves, through LDS.
 // Grid: 8 * ceil(n_atoms * nmat / 8) blocks in one dimension; block b takes
 // item (b % 8) * per + b / 8 = (matrix, atom) so that each XCD walks a
 // contiguous run of atoms of one matrix, whose gathers share x rows in L2.
+// Three waves per SIMD (168 registers; the compiler moves a few values to
+// scratch) measured faster than two without the spill (bulk3d, w = 128:
+// 2.40 against 3.24 ms for the column-chunk-outer form).
 template <bool RESIDUAL, int MAXR>
-__global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, const float2 *__restrict__ b,
+__global__ void __launch_bounds__(256, 3) k_atom_spmm(float2 *__restrict__ out, const float2 *__restrict__ b,
                                                   const int32_t *__restrict__ b_index, const float2 *__restrict__ x,
                                                   const int32_t *__restrict__ x_index, int w,
                                                   const float2 *__restrict__ h, const float2 *__restrict__ s,
@@ -63,18 +71,24 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
                                                   const int32_t *__restrict__ atom_off, int64_t n, int64_t nnz,
                                                   int n_atoms, int nmat, double *__restrict__ sumsq,
                                                   double *__restrict__ bsq) {
-    constexpr int CH = 64, NS = 8;
-    __shared__ double2 vals[CH][MAXR];
-    __shared__ double2 red[MAXR][32];
+    constexpr int CH = 64, XB = 8;  // column-list chunk; x values gathered per batch
+    __shared__ double2 vals[CH][MAXR];  // also the cross-wave reduction buffer
     __shared__ int32_t cidx[CH];
+    __shared__ int32_t rstart[MAXR];
     const int64_t total = (int64_t)n_atoms * nmat, per = (total + 7) / 8;
     const int64_t item = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (item >= total) return;
     const int m = (int)(item / n_atoms), a = (int)(item % n_atoms);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int half = lane >> 5, cl = lane & 31, stream = wave * 2 + half;
+    const int half = lane >> 5, cl = lane & 31, hw = wave * 2 + half;
     const int r0 = atom_off[a], na = atom_off[a + 1] - r0;
     const int p0 = indptr[r0], len = indptr[r0 + 1] - p0;
+    if (tid < na) rstart[tid] = indptr[r0 + tid];
+    const int nc32 = (w + 31) / 32;
+    const int ncc = nc32 >= 3 ? 4 : nc32;  // column chunks at a time: 1, 2 or 4
+    const int S = 8 / ncc;                 // streams per column chunk
+    const int cs = hw / S, st = hw % S;    // this half-wave's column chunk and stream
+    const int wpc = S / 2;                 // waves per column chunk
     const int k = kidx[m];
     const float2 *hk = h + (int64_t)k * nnz;
     const float2 *sk = RESIDUAL ? s + (int64_t)k * nnz : nullptr;
@@ -82,8 +96,8 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
     const float2 *xm = x + (int64_t)(RESIDUAL ? m : x_index[m]) * n * w;
     float2 *om = out + (int64_t)m * n * w;
     double local = 0.0, blocal = 0.0;
-    for (int c0 = 0; c0 < w; c0 += 32) {
-        const int c = c0 + cl;
+    for (int cg = 0; cg < w; cg += 32 * ncc) {
+        const int c = cg + 32 * cs + cl;
         const bool cv = c < w;
         double ar[MAXR], ai[MAXR];
 #pragma unroll
@@ -93,7 +107,7 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
             __syncthreads();
             for (int e = tid; e < cnt * na; e += 256) {
                 const int pp = e / na, r = e % na;
-                const int64_t pos = indptr[r0 + r] + q0 + pp;
+                const int64_t pos = rstart[r] + q0 + pp;
                 const float2 hv = hk[pos];
                 double vr = hv.x, vi = hv.y;
                 if (RESIDUAL) {
@@ -105,51 +119,65 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
             }
             for (int e = tid; e < cnt; e += 256) cidx[e] = indices[p0 + q0 + e];
             __syncthreads();
-            float2 xv[CH / NS];
+            // this stream's entries st, st + S, ..., gathered XB at a time,
+            // the next batch loading while the current one is multiplied
+            const int mine = cnt > st ? (cnt - st + S - 1) / S : 0;
+            float2 xv[XB], xn[XB];
+            auto gather = [&](float2 (&dst)[XB], int u0) {
 #pragma unroll
-            for (int u = 0; u < CH / NS; ++u) {
-                const int pp = stream + NS * u;
-                xv[u] = (cv && pp < cnt) ? xm[(int64_t)cidx[pp] * w + c] : make_float2(0.f, 0.f);
-            }
+                for (int u = 0; u < XB; ++u) {
+                    const int pp = st + S * (u0 + u);
+                    dst[u] = (cv && u0 + u < mine) ? xm[(int64_t)cidx[pp] * w + c] : make_float2(0.f, 0.f);
+                }
+            };
+            gather(xv, 0);
+            for (int u0 = 0; u0 < mine; u0 += XB) {
+                if (u0 + XB < mine) gather(xn, u0 + XB);
 #pragma unroll
-            for (int u = 0; u < CH / NS; ++u) {
-                const int pp = stream + NS * u;
-                if (pp >= cnt) break;
+                for (int u = 0; u < XB; ++u) {
+                    if (u0 + u >= mine) break;
+                    const int pp = st + S * (u0 + u);
 #pragma unroll
-                for (int r = 0; r < MAXR; ++r) {
-                    if (r < na) {
-                        const double2 v = vals[pp][r];
-                        ar[r] += v.x * xv[u].x - v.y * xv[u].y;
-                        ai[r] += v.x * xv[u].y + v.y * xv[u].x;
+                    for (int r = 0; r < MAXR; ++r) {
+                        if (r < na) {
+                            const double2 v = vals[pp][r];
+                            ar[r] += v.x * xv[u].x - v.y * xv[u].y;
+                            ai[r] += v.x * xv[u].y + v.y * xv[u].x;
+                        }
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < XB; ++u) xv[u] = xn[u];
             }
         }
-        // half-waves: lanes 32..63 into 0..31
+        // the two half-waves of a wave hold streams of the same column chunk
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             ar[r] += __shfl_down(ar[r], 32);
             ai[r] += __shfl_down(ai[r], 32);
         }
-        // waves 1..3 into wave 0
-        for (int src = 1; src < 4; ++src) {
+        // the other waves of the chunk into its first wave, through LDS
+        // (vals is free once every wave has left the products)
+        double2 *red = &vals[0][0];  // [column chunk][MAXR][32]
+        const int wc = wave % wpc;   // this wave's place among its chunk's waves
+        for (int src = 1; src < wpc; ++src) {
             __syncthreads();
-            if (wave == src && half == 0) {
+            if (wc == src && half == 0) {
 #pragma unroll
                 for (int r = 0; r < MAXR; ++r)
-                    if (r < na) red[r][cl] = make_double2(ar[r], ai[r]);
+                    if (r < na) red[(cs * MAXR + r) * 32 + cl] = make_double2(ar[r], ai[r]);
             }
             __syncthreads();
-            if (wave == 0 && half == 0) {
+            if (wc == 0 && half == 0) {
 #pragma unroll
                 for (int r = 0; r < MAXR; ++r)
                     if (r < na) {
-                        ar[r] += red[r][cl].x;
-                        ai[r] += red[r][cl].y;
+                        ar[r] += red[(cs * MAXR + r) * 32 + cl].x;
+                        ai[r] += red[(cs * MAXR + r) * 32 + cl].y;
                     }
             }
         }
-        if (wave == 0 && half == 0 && cv) {
+        if (wc == 0 && half == 0 && cv) {
 #pragma unroll
             for (int r = 0; r < MAXR; ++r) {
                 if (r >= na) break;
@@ -166,7 +194,7 @@ __global__ void __launch_bounds__(256) k_atom_spmm(float2 *__restrict__ out, con
             }
         }
     }
-    if (RESIDUAL && wave == 0) {
+    if (RESIDUAL && wave % wpc == 0) {
         for (int off = 16; off > 0; off >>= 1) {
             local += __shfl_down(local, off);
             blocal += __shfl_down(blocal, off);
