@@ -58,8 +58,9 @@ __global__ void __launch_bounds__(256) k_spmm(float2 *__restrict__ y, const floa
 // item (b % 8) * per + b / 8 = (matrix, atom) so that each XCD walks a
 // contiguous run of atoms of one matrix, whose gathers share x rows in L2.
 // Three waves per SIMD (168 registers; the compiler moves a few values to
-// scratch) measured faster than two without the spill (bulk3d, w = 128:
-// 2.40 against 3.24 ms for the column-chunk-outer form).
+// scratch) with the x gathers one batch ahead: on bulk3d 2.40 ms per call,
+// against 2.80 ms without the batch ahead (no spill) and 3.24 ms for the
+// form that looped over column chunks outside the column list.
 template <bool RESIDUAL, int MAXR>
 __global__ void __launch_bounds__(256, 3) k_atom_spmm(float2 *__restrict__ out, const float2 *__restrict__ b,
                                                   const int32_t *__restrict__ b_index, const float2 *__restrict__ x,
