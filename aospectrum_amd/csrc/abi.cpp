@@ -339,7 +339,10 @@ int aosx_shift_solve_v1(aosx_bundle *bundle, const aosx_shift_request_v1 *q, aos
         AOSX_HIP(hipStreamSynchronize(b.stream));
         double t3 = now();
         std::vector<float2> x((size_t)nb * n * W);
-        AOSX_HIP(hipMemcpy(x.data(), d_x, x.size() * 8, hipMemcpyDeviceToHost));
+        // copies on the bundle's stream, not the legacy one (which another
+        // thread's graph capture would refuse)
+        AOSX_HIP(hipMemcpyAsync(x.data(), d_x, x.size() * 8, hipMemcpyDeviceToHost, b.stream));
+        AOSX_HIP(hipStreamSynchronize(b.stream));
         for (int m = 0; m < nb; ++m)
             for (int64_t r = 0; r < n; ++r)
                 for (int c = 0; c < w; ++c) {
@@ -347,7 +350,8 @@ int aosx_shift_solve_v1(aosx_bundle *bundle, const aosx_shift_request_v1 *q, aos
                     q->solution[((size_t)m * n + r) * w + c] = {v.x, v.y};
                 }
         std::vector<int> repl(nb);
-        AOSX_HIP(hipMemcpy(repl.data(), d_repl, nb * sizeof(int), hipMemcpyDeviceToHost));
+        AOSX_HIP(hipMemcpyAsync(repl.data(), d_repl, nb * sizeof(int), hipMemcpyDeviceToHost, b.stream));
+        AOSX_HIP(hipStreamSynchronize(b.stream));
         for (int v : repl) res->replaced_pivots += v;
         res->refinement_steps = ir ? q->refinement_steps : 0;
         res->relative_residual = rel;

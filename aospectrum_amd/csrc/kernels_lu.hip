@@ -110,7 +110,9 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     const int m = blockIdx.x, tid = threadIdx.x;
     float *t = tile_ptr(tiles, mat_stride, m, tile_id[(int64_t)k * nt + k]);
     float amax = 0.f;
-    for (int e = tid; e < TILE2; e += 256) {
+#pragma unroll
+    for (int q = 0; q < TILE2 / 256; ++q) {  // unrolled: all loads in flight at once
+        const int e = q * 256 + tid;
         const int r = e / TILE, c = e % TILE;
         const float re = t[e], im = t[PLANE + e];
         Are[r * LDA + c] = re;
@@ -130,11 +132,36 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     // the trailing block with the unscaled multipliers times 1/pivot, and the
     // scaled column kk is written after the barrier (no later step reads it).
     // Thread (ti, tj) = (tid / 16, tid % 16) owns rows kk+1+ti+16a and
-    // columns kk+1+tj+16b.
+    // columns kk+1+tj+16b.  A step issues all of its LDS reads (pivot, its
+    // column and row entries, its targets) before it uses any and computes
+    // every new value before it stores any, so that it waits for LDS once:
+    // indices past the tile are clamped to the last row/column (in bounds,
+    // the values unused).
     const int ti = tid >> 4, tj = tid & 15;
     int local_replaced = 0;
     for (int kk = 0; kk < TILE; ++kk) {
+        int ic[4], jc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            ic[a] = min(kk + 1 + ti + 16 * a, TILE - 1);
+            jc[a] = min(kk + 1 + tj + 16 * a, TILE - 1);
+        }
         float pr = Are[kk * LDA + kk], pi = Aim[kk * LDA + kk];
+        float cr[4], ci[4], ur[4], ui[4], xr[4][4], xi[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            cr[a] = Are[ic[a] * LDA + kk];
+            ci[a] = Aim[ic[a] * LDA + kk];
+            ur[a] = Are[kk * LDA + jc[a]];
+            ui[a] = Aim[kk * LDA + jc[a]];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                xr[a][b] = Are[ic[a] * LDA + jc[b]];
+                xi[a][b] = Aim[ic[a] * LDA + jc[b]];
+            }
         const float mag = sqrtf(pr * pr + pi * pi);
         const bool rep = !(mag >= tau);
         if (rep) {
@@ -146,28 +173,27 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
         float lr[4], li[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const int i = kk + 1 + ti + 16 * a;
-            lr[a] = li[a] = 0.f;
-            if (i < TILE) {
-                const float ar = Are[i * LDA + kk], ai = Aim[i * LDA + kk];
-                lr[a] = ar * ir - ai * ii;
-                li[a] = ar * ii + ai * ir;
-            }
+            const bool in = kk + 1 + ti + 16 * a < TILE;
+            lr[a] = in ? cr[a] * ir - ci[a] * ii : 0.f;
+            li[a] = in ? cr[a] * ii + ci[a] * ir : 0.f;
         }
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = kk + 1 + tj + 16 * b;
-            if (j >= TILE) break;
-            const float ur = Are[kk * LDA + j], ui = Aim[kk * LDA + j];
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                const int i = kk + 1 + ti + 16 * a;
-                if (i < TILE) {
-                    Are[i * LDA + j] -= lr[a] * ur - li[a] * ui;
-                    Aim[i * LDA + j] -= lr[a] * ui + li[a] * ur;
+                xr[a][b] -= lr[a] * ur[b] - li[a] * ui[b];
+                xi[a][b] -= lr[a] * ui[b] + li[a] * ur[b];
+            }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = kk + 1 + ti + 16 * a, j = kk + 1 + tj + 16 * b;
+                if (i < TILE && j < TILE) {
+                    Are[i * LDA + j] = xr[a][b];
+                    Aim[i * LDA + j] = xi[a][b];
                 }
             }
-        }
         __syncthreads();
         if (tj == 0) {
 #pragma unroll
@@ -187,7 +213,9 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     }
     __syncthreads();
     if (tid == 0 && local_replaced) atomicAdd(replaced + m, local_replaced);
-    for (int e = tid; e < TILE2; e += 256) {
+#pragma unroll
+    for (int q = 0; q < TILE2 / 256; ++q) {  // unrolled: all loads in flight at once
+        const int e = q * 256 + tid;
         const int r = e / TILE, c = e % TILE;
         t[e] = Are[r * LDA + c];
         t[PLANE + e] = Aim[r * LDA + c];
@@ -201,25 +229,35 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     // factored tile, and the four lanes of a column (row) sit in one wave, so
     // the recurrences need no workgroup barrier: LDS operations of one wave
     // complete in issue order.  Iteration i takes row i of X and column i of Y.
+    // Both recurrences of iteration i run over the same indices j = col +
+    // part + 4t < i (row = col, jj = i); four t are loaded before any is
+    // used, with indices clamped to the tile (the extra values unused).
     const int col = tid >> 2, part = tid & 3, row = col;
     for (int i = 0; i < TILE; ++i) {
         float sr = 0.f, si = 0.f, tr = 0.f, ti2 = 0.f;
-        if (i > col) {
-            for (int j = col + part; j < i; j += 4) {
-                const float ar = Are[i * LDA + j], ai = Aim[i * LDA + j];
-                const float xr = Lre[j * LDA + col], xi = Lim[j * LDA + col];
-                sr += ar * xr - ai * xi;
-                si += ar * xi + ai * xr;
-            }
-        }
         const int jj = i;  // column of Y
-        if (jj > row) {
-            for (int q = row + part; q < jj; q += 4) {
-                const float yr = Ure[row * LDA + q], yi = Uim[row * LDA + q];
-                const float ur = Are[q * LDA + jj], ui = Aim[q * LDA + jj];
-                tr += yr * ur - yi * ui;
-                ti2 += yr * ui + yi * ur;
+        for (int t0 = col + part; t0 < i; t0 += 16) {
+            float ar[4], ai[4], xr[4], xi[4], yr[4], yi[4], ur[4], ui[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = min(t0 + 4 * u, TILE - 1);
+                ar[u] = Are[i * LDA + j];
+                ai[u] = Aim[i * LDA + j];
+                xr[u] = Lre[j * LDA + col];
+                xi[u] = Lim[j * LDA + col];
+                yr[u] = Ure[row * LDA + j];
+                yi[u] = Uim[row * LDA + j];
+                ur[u] = Are[j * LDA + jj];
+                ui[u] = Aim[j * LDA + jj];
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + 4 * u < i) {
+                    sr += ar[u] * xr[u] - ai[u] * xi[u];
+                    si += ar[u] * xi[u] + ai[u] * xr[u];
+                    tr += yr[u] * ur[u] - yi[u] * ui[u];
+                    ti2 += yr[u] * ui[u] + yi[u] * ur[u];
+                }
         }
         sr += __shfl_xor(sr, 1);
         si += __shfl_xor(si, 1);
@@ -250,7 +288,9 @@ __global__ void __launch_bounds__(256) k_getrf_diag(float *__restrict__ tiles, i
     }
     __syncthreads();  // X and Y are complete before the copy-out below
     float *dv = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
-    for (int e = tid; e < TILE2; e += 256) {
+#pragma unroll
+    for (int q = 0; q < TILE2 / 256; ++q) {  // unrolled: all loads in flight at once
+        const int e = q * 256 + tid;
         const int r = e / TILE, c = e % TILE;
         dv[e] = Lre[r * LDA + c];
         dv[PLANE + e] = Lim[r * LDA + c];

@@ -1542,7 +1542,10 @@ void Bundle::band(const CissParams &p, const double *kpts, const float2 *h_in, c
             report(kb, 0, g, ngroups);
         }
         std::vector<int> repl(slots);
-        AOSX_HIP(hipMemcpy(repl.data(), d_repl, slots * sizeof(int), hipMemcpyDeviceToHost));
+        // on this bundle's (non-blocking) stream: a copy on the legacy stream
+        // is refused while another thread captures a graph
+        AOSX_HIP(hipMemcpyAsync(repl.data(), d_repl, slots * sizeof(int), hipMemcpyDeviceToHost, stream));
+        AOSX_HIP(hipStreamSynchronize(stream));
         for (int v : repl) out.replaced_pivots += v;
         // subspace stage
         Subspace sub;
