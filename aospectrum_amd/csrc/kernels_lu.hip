@@ -640,12 +640,13 @@ __global__ void __launch_bounds__(256, 2) k_panel_split(float *__restrict__ tile
     const int32_t id = lower ? tile_id[(int64_t)other * nt + k] : tile_id[(int64_t)k * nt + other];
     float *t = tile_ptr(tiles, mat_stride, m, id);
     const float *d = dinv + ((int64_t)m * nt + k) * (2 * TILE_FLOATS);
-    if (lower) {
-        stage_split(t, As, false, false, false, tid);               // L(i,k)
-        stage_split(d + TILE_FLOATS, Bs, true, false, false, tid);  // inv(U_kk)
-    } else {
-        stage_split(d, As, false, false, false, tid);               // inv(L_kk)
-        stage_split(t, Bs, true, false, false, tid);                // U(k,j)
+    {
+        // both operand tiles load before either is split: one memory round
+        // trip, not two
+        const TileBlock ta = tile_block_load(lower ? t : d, tid);                // L(i,k) or inv(L_kk)
+        const TileBlock tb = tile_block_load(lower ? d + TILE_FLOATS : t, tid);  // inv(U_kk) or U(k,j)
+        stage_split_block(ta, As, false, false, false, tid);
+        stage_split_block(tb, Bs, true, false, false, tid);
     }
     __syncthreads();
     floatx16 cre = {}, cim = {};
